@@ -1,0 +1,110 @@
+#!/usr/bin/env python3
+"""Throughput of the D3Q19 TGV time step on one MI355X, as one JSON line.
+
+Workload: uniform periodic D3Q19 BGK Taylor-Green vortex, 512^3, FP32 populations, Re 1600, u0 0.05
+(two population buffers, 20.4 GB), the lattice's own closed-form initial state. A step is one fused
+pull stream-collide launch over the whole lattice; W untimed steps, then K steps timed with HIP events
+on the context's stream (inside the library, jp_bench). value = GLUPS (1e9 cell updates / s).
+
+roofline: the step kernel moves 19 x 4 B read + 19 x 4 B written = 152 B per cell update; achieved =
+152 B x N^3 / (average step duration), against MI355X's 8 TB/s HBM3E peak. traffic comes from the
+committed rocprofv3 counter summary for this workload (profiles/), scaled by a copy kernel's counters.
+cpu_baseline: the CPU oracle (the reference step restated in OpenMP C++) on the host cores, on a
+bounded sample of the same lattice update (128^3, a few tens of steps).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jp-solver_amd"))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0
+METRIC = "GLUPS + % HBM roofline, uniform & AMR TGV, FP32 vs FP16-store at matched err"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json")
+
+
+def cpu_baseline(target_s=15.0):
+    import oracle
+
+    n = 128
+    o = oracle.Uniform(n, re=1600.0, u0=0.05)
+    o.init()
+    t1 = o.time_steps(2)
+    steps = int(max(4, min(400, target_s / max(t1 / 2, 1e-6))))
+    t = o.time_steps(steps)
+    glups = n ** 3 * steps / t / 1e9
+    return {"value": glups, "unit": "GLUPS", "cores": oracle.lib().orc_threads(), "kind": "port",
+            "sample": f"oracle uniform {n}^3 FP32 Re 1600, {steps} steps in {t:.1f} s "
+                      f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
+    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import jpabi
+
+    prec = jpabi.PREC_NAMES[args.prec]
+    n = args.n
+    copy_gbps, _ = jpabi.probe_copy(4 << 30, iters=10)
+    g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=prec)
+    g.init()
+    print(f"# uniform {n}^3 {args.prec}: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr,
+          flush=True)
+    r = g.bench(args.steps, warmup=args.warmup, graph=bool(args.graph))
+    E, ens, mass = g.diag()
+    g.close()
+    bpu = r["bytes_per_update"]
+    achieved = bpu * n ** 3 / (r["ms_per_step"] * 1e-3) / 1e9  # GB/s of the step kernel
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.prec == "fp32" and n == 512:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        traffic = t.get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": r["glups"],
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp16": "f16-store/f32", "bf16": "bf16-store/f32", "fp64": "f64"}[args.prec],
+        "data": "synthetic (closed-form TGV initial state, no dataset)",
+        "config": {"workload": f"uniform D3Q19 BGK TGV {n}^3 {args.prec.upper()} populations, Re 1600, u0 0.05, "
+                               f"{args.steps} timed steps", "n": n, "re": 1600.0, "u0": 0.05,
+                   "precision": args.prec, "graph": bool(args.graph)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
+        "extra": {"copy_kernel_gbps": copy_gbps, "bytes_per_update": bpu, "pool_gb": r["pool_gb"],
+                  "diag_after": [E, ens, mass]},
+    }
+    if not args.no_cpu:
+        t0 = time.time()
+        line["cpu_baseline"] = cpu_baseline()
+        line["extra"]["cpu_baseline_wall_s"] = time.time() - t0
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

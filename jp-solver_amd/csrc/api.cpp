@@ -1,0 +1,353 @@
+// C-ABI layer (include/jp.h): contexts, device memory, orchestration of the time step, diagnostics,
+// benchmarking and the test hooks. Host code only; kernels live in the .hip translation units.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jp.h"
+#include "ctx.hpp"
+
+using namespace jp;
+
+namespace jp {
+
+int fail(jp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int ctx_alloc(jp_ctx* c, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(c, e == hipErrorOutOfMemory ? JP_ERR_NOMEM : JP_ERR_HIP,
+                std::string("hipMalloc(") + what + ", " + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+  }
+  c->allocs.emplace_back(*p, bytes);
+  c->pool_bytes += bytes;
+  return JP_OK;
+}
+
+void ctx_free(jp_ctx* c, void* p) {
+  if (!p) return;
+  for (auto it = c->allocs.begin(); it != c->allocs.end(); ++it)
+    if (it->first == p) {
+      c->pool_bytes -= it->second;
+      c->allocs.erase(it);
+      break;
+    }
+  (void)hipFree(p);
+}
+
+double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 / u0))); }
+
+int drop_graph(jp_ctx* c) {
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return JP_OK;
+}
+
+}  // namespace jp
+
+// ---------------------------------------------------------------------------------------------------
+// uniform
+// ---------------------------------------------------------------------------------------------------
+static int uniform_step_once(jp_ctx* c) {
+  const float enc = float(c->d.scale[0]), dec = float(1.0 / c->d.scale[0]);
+  HCK(c, dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->variant, c->st));
+  std::swap(c->c0a, c->c0b);
+  return JP_OK;
+}
+
+static int ensure_diag_buffers(jp_ctx* c, int nfield) {
+  const size_t fb = size_t(nfield) * nfield * nfield * 4 * (c->prec == JP_PREC_FP64 ? 8 : 4);
+  if (c->field_bytes < fb) {
+    ctx_free(c, c->field);
+    c->field = nullptr;
+    c->field_bytes = 0;
+    if (int r = ctx_alloc(c, &c->field, fb, "diagnostic field")) return r;
+    c->field_bytes = fb;
+  }
+  if (!c->partial) {
+    void* p;
+    if (int r = ctx_alloc(c, &p, size_t(kDiagBlocks) * 3 * 8 + 64, "diagnostic partials")) return r;
+    c->partial = static_cast<double*>(p);
+  }
+  return JP_OK;
+}
+
+extern "C" {
+
+const char* jp_version(void) { return "jp-solver-mi355x 0.1 (gfx950)"; }
+
+int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
+  if (!p || !out) return JP_ERR_ARG;
+  *out = nullptr;
+  auto* c = new jp_ctx();
+  c->kind = 0;
+  if (p->n < 8 || p->n % MB != 0 || p->n > 1280) {
+    *out = c;
+    return fail(c, JP_ERR_ARG, "N%MB: n must be a positive multiple of 8 (and at most 1280)");
+  }
+  if (p->prec < 0 || p->prec > 3 || !(p->u0 > 0) || !(p->re > 0)) {
+    *out = c;
+    return fail(c, JP_ERR_ARG, "bad precision, u0 or Re");
+  }
+  c->device = p->device;
+  c->prec = p->prec;
+  c->u0 = p->u0;
+  c->re = p->re;
+  for (int i = 0; i < 3; ++i) c->phase[i] = p->phase[i];
+  *out = c;
+  HCK(c, hipSetDevice(c->device));
+  HCK(c, hipStreamCreate(&c->st));
+  const int N = p->n;
+  // src/tgv3d_uniform.cu:151-152
+  c->d.n = N;
+  c->d.prec = p->prec;
+  c->d.nu = p->u0 * double(N) / (2.0 * M_PI * p->re);
+  c->d.tau0 = 3.0 * c->d.nu + 0.5;
+  c->d.omega0 = (p->prec == JP_PREC_FP64) ? 1.0 / c->d.tau0 : double(float(1.0 / c->d.tau0));
+  c->d.kf = 2.0 * M_PI / double(N);
+  c->d.cells = long(N) * N * N;
+  const double s = (p->prec == JP_PREC_FP16 || p->prec == JP_PREC_BF16) ? (p->scale > 0 ? p->scale : default_scale(p->u0)) : 1.0;
+  if (s != std::exp2(std::round(std::log2(s)))) return fail(c, JP_ERR_ARG, "scale must be a power of two");
+  c->d.scale[0] = c->d.scale[1] = c->d.scale[2] = s;
+  c->g0 = make_dense(N, 0);
+  const size_t bytes = size_t(Q) * c->g0.plane * prec_bytes(c->prec);
+  if (int r = ctx_alloc(c, &c->c0a, bytes, "populations a")) return r;
+  if (int r = ctx_alloc(c, &c->c0b, bytes, "populations b")) return r;
+  return JP_OK;
+}
+
+int jp_init(jp_ctx* c) {
+  if (!c) return JP_ERR_ARG;
+  HCK(c, hipSetDevice(c->device));
+  if (c->kind == 1) return amr_init(c);
+  HCK(c, dense_init(c->prec, c->c0a, c->g0, c->u0, c->d.kf, c->phase, float(c->d.scale[0]), c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  c->initialised = true;
+  return JP_OK;
+}
+
+int jp_step(jp_ctx* c, long n) {
+  if (!c) return JP_ERR_ARG;
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_step before jp_init");
+  for (long s = 0; s < n; ++s) {
+    int r = c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
+    if (r) return r;
+  }
+  return JP_OK;
+}
+
+int jp_diag(jp_ctx* c, double out[3]) {
+  if (!c || !out) return JP_ERR_ARG;
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_diag before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  const int nf = c->kind == 1 ? c->amr.N2 : c->d.n;
+  if (int r = ensure_diag_buffers(c, nf)) return r;
+  if (c->kind == 1) {
+    if (int r = amr_composite(c, c->field)) return r;
+  } else {
+    HCK(c, dense_macro(c->prec, c->c0a, c->g0, float(1.0 / c->d.scale[0]), c->field, c->st));
+  }
+  HCK(c, field_diag(c->prec == JP_PREC_FP64, c->kind == 0, c->field, nf, c->partial, kDiagBlocks,
+                    c->partial + kDiagBlocks * 3, c->st));
+  HCK(c, hipMemcpyAsync(out, c->partial + kDiagBlocks * 3, 24, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+
+int jp_get_derived(jp_ctx* c, jp_derived* d) {
+  if (!c || !d) return JP_ERR_ARG;
+  *d = c->d;
+  return JP_OK;
+}
+
+int jp_population_bytes(jp_ctx* c, int level, size_t* bytes) {
+  if (!c || !bytes) return JP_ERR_ARG;
+  long tiles;
+  if (level == 0) tiles = c->d.cells / MB3;
+  else if (c->kind == 1 && level == 1) tiles = c->amr.nf1;
+  else if (c->kind == 1 && level == 2) tiles = c->amr.nf2;
+  else return fail(c, JP_ERR_ARG, "bad level");
+  *bytes = size_t(tiles) * Q * MB3 * prec_bytes(c->prec);
+  return JP_OK;
+}
+
+static int level_buffer(jp_ctx* c, int level, void** p) {
+  if (level == 0) *p = c->c0a;
+  else if (c->kind == 1 && level == 1) *p = c->amr.f1a;
+  else if (c->kind == 1 && level == 2) *p = c->amr.f2a;
+  else return fail(c, JP_ERR_ARG, "bad level");
+  return JP_OK;
+}
+
+int jp_get_populations(jp_ctx* c, int level, void* dst, size_t bytes) {
+  if (!c || !dst) return JP_ERR_ARG;
+  size_t need;
+  if (int r = jp_population_bytes(c, level, &need)) return r;
+  if (bytes != need) return fail(c, JP_ERR_ARG, "population buffer size mismatch");
+  HCK(c, hipSetDevice(c->device));
+  void* src;
+  if (int r = level_buffer(c, level, &src)) return r;
+  if (level == 0) {
+    void* tmp;
+    if (int r = ctx_alloc(c, &tmp, need, "tile staging")) return r;
+    hipError_t e = dense_to_tiles(c->prec, src, c->g0, tmp, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, need, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    ctx_free(c, tmp);
+    if (e != hipSuccess) return fail(c, JP_ERR_HIP, hipGetErrorString(e));
+    return JP_OK;
+  }
+  if (need) {
+    HCK(c, hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+  }
+  return JP_OK;
+}
+
+int jp_set_populations(jp_ctx* c, int level, const void* srcp, size_t bytes) {
+  if (!c || !srcp) return JP_ERR_ARG;
+  size_t need;
+  if (int r = jp_population_bytes(c, level, &need)) return r;
+  if (bytes != need) return fail(c, JP_ERR_ARG, "population buffer size mismatch");
+  HCK(c, hipSetDevice(c->device));
+  void* dstb;
+  if (int r = level_buffer(c, level, &dstb)) return r;
+  drop_graph(c);
+  if (level == 0) {
+    void* tmp;
+    if (int r = ctx_alloc(c, &tmp, need, "tile staging")) return r;
+    hipError_t e = hipMemcpyAsync(tmp, srcp, need, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = tiles_to_dense(c->prec, tmp, c->g0, dstb, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    ctx_free(c, tmp);
+    if (e != hipSuccess) return fail(c, JP_ERR_HIP, hipGetErrorString(e));
+  } else if (need) {
+    HCK(c, hipMemcpyAsync(dstb, srcp, need, hipMemcpyHostToDevice, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+  }
+  c->initialised = true;
+  return JP_OK;
+}
+
+int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result* res) {
+  if (!c || !res || steps < 1 || warmup < 0) return JP_ERR_ARG;
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_bench before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  if (int r = jp_step(c, warmup)) return r;
+  HCK(c, hipStreamSynchronize(c->st));
+  hipEvent_t e0, e1;
+  HCK(c, hipEventCreate(&e0));
+  HCK(c, hipEventCreate(&e1));
+  long done = 0;
+  int rc = JP_OK;
+  if (use_graph) {
+    // two steps per graph so the ping-pong buffers return to their starting roles
+    if (steps % 2) steps += 1;
+    drop_graph(c);
+    hipGraph_t g;
+    HCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    rc = jp_step(c, 2);
+    hipError_t ec = hipStreamEndCapture(c->st, &g);
+    if (rc) return rc;
+    if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
+    HCK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    HCK(c, hipGraphUpload(c->gexec, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+    HCK(c, hipEventRecord(e0, c->st));
+    for (long r = 0; r < steps / 2; ++r) HCK(c, hipGraphLaunch(c->gexec, c->st));
+    HCK(c, hipEventRecord(e1, c->st));
+    done = steps;
+  } else {
+    HCK(c, hipEventRecord(e0, c->st));
+    rc = jp_step(c, steps);
+    if (rc) return rc;
+    HCK(c, hipEventRecord(e1, c->st));
+    done = steps;
+  }
+  HCK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HCK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double per_step_work = double(c->d.cells);
+  if (c->kind == 1) per_step_work = double(c->amr.cnb0 + 2L * c->amr.nf1 + 4L * c->amr.nf2) * MB3;
+  res->steps = done;
+  res->ms_total = ms;
+  res->ms_per_step = ms / double(done);
+  res->work = per_step_work * double(done);
+  res->glups = res->work / (double(ms) * 1e6);
+  res->bytes_per_update = 38.0 * double(prec_bytes(c->prec));
+  res->gbps = res->glups * res->bytes_per_update;
+  res->pool_gb = double(c->pool_bytes) / double(1ull << 30);
+  return JP_OK;
+}
+
+int jp_sync(jp_ctx* c) {
+  if (!c) return JP_ERR_ARG;
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+
+const char* jp_last_error(const jp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void jp_destroy(jp_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto& a : c->allocs) (void)hipFree(a.first);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_iter) {
+  if (!gbps || iters < 1 || bytes < 16) return JP_ERR_ARG;
+  bytes &= ~size_t(15);
+  if (hipSetDevice(device) != hipSuccess) return JP_ERR_HIP;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st;
+  if (hipStreamCreate(&st) != hipSuccess) return JP_ERR_HIP;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    (void)hipStreamDestroy(st);
+    return JP_ERR_NOMEM;
+  }
+  int rc = JP_OK;
+  hipEvent_t e0, e1;
+  (void)hipMemsetAsync(a, 0, bytes, st);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  if (copy_probe(a, b, bytes, st) != hipSuccess) rc = JP_ERR_HIP;
+  (void)hipEventRecord(e0, st);
+  for (int i = 0; i < iters && rc == JP_OK; ++i)
+    if (copy_probe(a, b, bytes, st) != hipSuccess) rc = JP_ERR_HIP;
+  (void)hipEventRecord(e1, st);
+  if (hipEventSynchronize(e1) != hipSuccess) rc = JP_ERR_HIP;
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * double(bytes) * iters / (double(ms) * 1e6);
+  if (ms_per_iter) *ms_per_iter = ms / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipStreamDestroy(st);
+  return rc;
+}
+
+}  // extern "C"

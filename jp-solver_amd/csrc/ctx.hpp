@@ -1,0 +1,77 @@
+// Context object behind the opaque jp_ctx handle of include/jp.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/jp.h"
+#include "internal.hpp"
+
+namespace jp {
+
+// Device state of the 3-level block AMR (tables follow src/tgv3d_amr.cu:692-709).
+struct AmrState {
+  int N = 0, cgx = 0, cnb0 = 0, l1gx = 0, N1 = 0, N2 = 0, NG1 = 0;
+  int l1_mode = 0, l2_mode = 0, skipprep = 0, skiprestr = 0;
+  double frac1 = 0.35, frac2 = 0.35, budget = 0;
+  int cap_nref0 = 0, cap1 = 0, cap_nref1 = 0, cap2 = 0;
+  long hw_nf1 = 0, hw_nf2 = 0;
+  int nref0 = 0, nf1 = 0, nref1 = 0, nf2 = 0, n_skin0 = 0, n_skin1 = 0;
+  void *f1a = nullptr, *f1b = nullptr, *f2a = nullptr, *f2b = nullptr;  // stored populations
+  void *prep0 = nullptr, *prep1 = nullptr, *gh1 = nullptr, *gh2 = nullptr;  // working-type values
+  int *ref0 = nullptr, *fof0 = nullptr, *rl0 = nullptr, *l1pcb = nullptr, *l1oct = nullptr, *f1pos = nullptr;
+  int *l1grid = nullptr, *ref1 = nullptr, *fof1 = nullptr, *rl1 = nullptr, *l2pl1 = nullptr, *l2oct = nullptr;
+  int *skin0 = nullptr, *skin1 = nullptr, *o_ref0 = nullptr, *o_fof0 = nullptr, *o_l1grid = nullptr;
+  int *o_ref1 = nullptr, *o_fof1 = nullptr, *rank = nullptr, *deep = nullptr, *flags = nullptr;
+  int *nb1 = nullptr, *nb2 = nullptr;  // 27 same-level neighbour tiles per fine tile (-1 = absent)
+  float *sen0 = nullptr, *sen1 = nullptr;
+  int* dcnt = nullptr;  // device-resident counts and flags (see amr.hip)
+  int* hcnt = nullptr;  // pinned host mirror of dcnt
+};
+
+}  // namespace jp
+
+struct jp_ctx {
+  int kind = 0;  // 0 uniform, 1 AMR
+  int device = 0, prec = 0, variant = 0;
+  bool initialised = false;
+  hipStream_t st = nullptr;
+  std::string err;
+  jp_derived d{};
+  double u0 = 0.05, re = 1600, phase[3] = {0, 0, 0};
+  jp::Dense g0{};
+  void *c0a = nullptr, *c0b = nullptr;  // dense level (uniform lattice or AMR level 0)
+  void* field = nullptr;
+  size_t field_bytes = 0;
+  double* partial = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<std::pair<void*, size_t>> allocs;
+  size_t pool_bytes = 0;
+  jp::AmrState amr;
+};
+
+namespace jp {
+int fail(jp_ctx* c, int code, const std::string& msg);
+int ctx_alloc(jp_ctx* c, void** p, size_t bytes, const char* what);
+template <class T> int ctx_alloc_t(jp_ctx* c, T** p, size_t count, const char* what) {
+  void* v = nullptr;
+  int r = ctx_alloc(c, &v, count * sizeof(T), what);
+  *p = static_cast<T*>(v);
+  return r;
+}
+void ctx_free(jp_ctx* c, void* p);
+int drop_graph(jp_ctx* c);
+double default_scale(double u0);
+// AMR orchestration (amr_host.cpp)
+int amr_init(jp_ctx* c);
+int amr_base_step(jp_ctx* c, hipStream_t st);
+int amr_composite(jp_ctx* c, void* field);
+}  // namespace jp
+
+#define HCK(c, x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) return ::jp::fail((c), JP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)

@@ -1,0 +1,48 @@
+// Internal launch interface between the C-ABI layer (api.cpp) and the kernel translation units.
+// Populations of a dense periodic lattice (the uniform lattice and AMR level 0) are stored as 19
+// x-fastest planes: f[q*plane + (z*N + y)*N + x]. Scattered AMR tiles (levels 1 and 2) are stored
+// tile-major: f[(tile*19 + q)*512 + cell], cell = (k*8 + j)*8 + i.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "lattice.hpp"
+
+namespace jp {
+
+struct Dense {
+  uint32_t N;      // edge
+  uint64_t plane;  // elements between consecutive q planes (>= N^3)
+  FastDiv divN;
+};
+inline Dense make_dense(int N, uint64_t pad) {
+  Dense d;
+  d.N = uint32_t(N);
+  d.plane = uint64_t(N) * N * N + pad;
+  d.divN = make_fastdiv(uint32_t(N));
+  return d;
+}
+inline size_t prec_bytes(int prec) { return prec == 3 ? 8 : (prec == 0 ? 4 : 2); }
+
+// ---- dense lattice (uniform.hip) ----
+hipError_t dense_init(int prec, void* f, const Dense& g, double u0, double kf, const double* phase, float enc,
+                      hipStream_t st);
+hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
+                      int variant, hipStream_t st);
+// macro field (rho, u, v, w): float4 for FP32/FP16/BF16 storage, double4 for FP64 storage
+hipError_t dense_macro(int prec, const void* f, const Dense& g, float dec, void* field, hipStream_t st);
+// E, enstrophy, mass sums of a macro field; diff_in_field_type selects the uniform driver's central
+// differences (taken in the field's type) over the AMR reduction's (taken in double)
+hipError_t field_diag(bool dbl_field, bool diff_in_field_type, const void* field, int N, double* partial,
+                      int nblocks, double* out3, hipStream_t st);
+constexpr int kDiagBlocks = 1024;
+hipError_t copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
+// tile-major <-> dense plane conversion for the test hooks (raw elements, any precision)
+hipError_t dense_to_tiles(int prec, const void* f, const Dense& g, void* tiles, hipStream_t st);
+hipError_t tiles_to_dense(int prec, const void* tiles, const Dense& g, void* f, hipStream_t st);
+// max |f - w_q| over the lattice (FP32 state), used to pick the shifted-store scale
+hipError_t fill_i32(int* p, int v, size_t n, hipStream_t st);
+
+}  // namespace jp

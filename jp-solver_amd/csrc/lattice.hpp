@@ -1,0 +1,207 @@
+// D3Q19 lattice core for gfx950 device code: velocity set, weights, equilibrium, Hermite
+// regularisation and the population storage forms. Everything direction-dependent is resolved at
+// compile time (each q is a template constant), so zero velocity components cost no instructions.
+//
+// Arithmetic contract: kernels are compiled with -ffp-contract=off and IEEE division/sqrt, and every
+// expression keeps the operation order of the reference method (src/tgv3d_uniform.cu:35-38, 55-60;
+// src/tgv3d_amr.cu:52-67). Dropped terms are only exact zeros (c = 0 components), which can change the
+// sign of a zero intermediate but never a non-zero result.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+namespace jp {
+
+constexpr int Q = 19, MB = 8, MB3 = 512, HALF = 4;
+
+// q order {0; +x,-x,+y,-y,+z,-z; xy(4); xz(4); yz(4)} as in src/tgv3d_uniform.cu:28-30.
+__host__ __device__ constexpr int CX(int q) {
+  constexpr int t[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  return t[q];
+}
+__host__ __device__ constexpr int CY(int q) {
+  constexpr int t[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  return t[q];
+}
+__host__ __device__ constexpr int CZ(int q) {
+  constexpr int t[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  return t[q];
+}
+template <class R> __host__ __device__ constexpr R W(int q) {
+  return q == 0 ? R(1) / R(3) : (q < 7 ? R(1) / R(18) : R(1) / R(36));
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1.
+template <class F, int... I> __host__ __device__ inline void sfor_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __host__ __device__ inline void sfor(F&& f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// c_q . u summed over the non-zero components in x, y, z order.
+template <int q, class R> __device__ inline R cdot(R ux, R uy, R uz) {
+  constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+  R acc = R(0);
+  if constexpr (cx != 0) acc = cx > 0 ? ux : -ux;
+  if constexpr (cy != 0) {
+    R t = cy > 0 ? uy : -uy;
+    if constexpr (cx != 0) acc = acc + t;
+    else acc = t;
+  }
+  if constexpr (cz != 0) {
+    R t = cz > 0 ? uz : -uz;
+    if constexpr (cx != 0 || cy != 0) acc = acc + t;
+    else acc = t;
+  }
+  return acc;
+}
+
+// Moments rho, rho*u accumulated in q order (src/tgv3d_uniform.cu:55-57).
+template <class R> struct Mom {
+  R rho, ux, uy, uz;
+};
+template <class R> __device__ inline Mom<R> moments(const R* f) {
+  Mom<R> m;
+  m.rho = f[0];
+  m.ux = R(0);
+  m.uy = R(0);
+  m.uz = R(0);
+  // the first non-zero component of each axis is q = 1 (+x), 3 (+y), 5 (+z)
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if constexpr (q > 0) m.rho = m.rho + f[q];
+    if constexpr (CX(q) != 0) {
+      if constexpr (q == 1) m.ux = f[q];
+      else m.ux = CX(q) > 0 ? m.ux + f[q] : m.ux - f[q];
+    }
+    if constexpr (CY(q) != 0) {
+      if constexpr (q == 3) m.uy = f[q];
+      else m.uy = CY(q) > 0 ? m.uy + f[q] : m.uy - f[q];
+    }
+    if constexpr (CZ(q) != 0) {
+      if constexpr (q == 5) m.uz = f[q];
+      else m.uz = CZ(q) > 0 ? m.uz + f[q] : m.uz - f[q];
+    }
+  });
+  return m;
+}
+
+// Equilibrium factors shared by all q of one cell: w_q*rho per weight class and 1.5|u|^2.
+template <class R> struct EqPre {
+  R wr[3], t15, ux, uy, uz;
+};
+template <class R> __device__ inline EqPre<R> eq_pre(R rho, R ux, R uy, R uz) {
+  EqPre<R> e;
+  e.wr[0] = W<R>(0) * rho;
+  e.wr[1] = W<R>(1) * rho;
+  e.wr[2] = W<R>(7) * rho;
+  e.t15 = R(1.5) * (ux * ux + uy * uy + uz * uz);
+  e.ux = ux;
+  e.uy = uy;
+  e.uz = uz;
+  return e;
+}
+// w_q rho (1 + cu + cu^2/2 - 1.5 u^2), cu = 3 c.u (src/tgv3d_uniform.cu:35-38).
+template <int q, class R> __device__ inline R feq(const EqPre<R>& e) {
+  constexpr int cls = q == 0 ? 0 : (q < 7 ? 1 : 2);
+  if constexpr (q == 0) {
+    return e.wr[0] * (R(1) - e.t15);
+  } else {
+    R cu = R(3) * cdot<q>(e.ux, e.uy, e.uz);
+    return e.wr[cls] * (R(1) + cu + R(0.5) * cu * cu - e.t15);
+  }
+}
+
+// Second-order Hermite projection of a non-equilibrium vector (src/tgv3d_amr.cu:56-67).
+template <class R> __device__ inline void regularize(R* n) {
+  R Pxx = R(0), Pyy = R(0), Pzz = R(0), Pxy = R(0), Pxz = R(0), Pyz = R(0);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const R v = n[q];
+    Pxx += R(CX(q) * CX(q)) * v;
+    Pyy += R(CY(q) * CY(q)) * v;
+    Pzz += R(CZ(q) * CZ(q)) * v;
+    Pxy += R(CX(q) * CY(q)) * v;
+    Pxz += R(CX(q) * CZ(q)) * v;
+    Pyz += R(CY(q) * CZ(q)) * v;
+  });
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const R third = R(1) / R(3);
+    const R Qxx = R(CX(q) * CX(q)) - third, Qyy = R(CY(q) * CY(q)) - third, Qzz = R(CZ(q) * CZ(q)) - third;
+    const R Qxy = R(CX(q) * CY(q)), Qxz = R(CX(q) * CZ(q)), Qyz = R(CY(q) * CZ(q));
+    n[q] = R(4.5) * W<R>(q) * (Qxx * Pxx + Qyy * Pyy + Qzz * Pzz + R(2) * (Qxy * Pxy + Qxz * Pxz + Qyz * Pyz));
+  });
+}
+
+// ---- storage forms ----
+// A form maps the stored element of direction q to the working type and back. FP16/BF16 hold the
+// shifted value (f - w_q) * enc with enc a power of two; dec = 1/enc exactly.
+struct FormF32 {
+  using raw = float;
+  using real = float;
+  static constexpr int id = 0;
+  template <int q> __device__ static inline real load(raw r, float) { return r; }
+  template <int q> __device__ static inline raw store(real f, float) { return f; }
+};
+struct FormF64 {
+  using raw = double;
+  using real = double;
+  static constexpr int id = 3;
+  template <int q> __device__ static inline real load(raw r, float) { return r; }
+  template <int q> __device__ static inline raw store(real f, float) { return f; }
+};
+struct FormF16 {
+  using raw = _Float16;
+  using real = float;
+  static constexpr int id = 1;
+  template <int q> __device__ static inline real load(raw r, float dec) { return float(r) * dec + W<float>(q); }
+  template <int q> __device__ static inline raw store(real f, float enc) {
+    return static_cast<_Float16>((f - W<float>(q)) * enc);
+  }
+};
+__device__ inline uint16_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+struct FormBF16 {
+  using raw = uint16_t;
+  using real = float;
+  static constexpr int id = 2;
+  template <int q> __device__ static inline real load(raw r, float dec) {
+    return __uint_as_float(uint32_t(r) << 16) * dec + W<float>(q);
+  }
+  template <int q> __device__ static inline raw store(real f, float enc) { return bf16_rne((f - W<float>(q)) * enc); }
+};
+
+// Runtime-q decode for kernels that gather one direction at a time.
+template <class S> __device__ inline typename S::real load_rt(typename S::raw r, int q, float dec) {
+  if constexpr (S::id == 1) return float(r) * dec + (q == 0 ? W<float>(0) : (q < 7 ? W<float>(1) : W<float>(7)));
+  else if constexpr (S::id == 2)
+    return __uint_as_float(uint32_t(r) << 16) * dec + (q == 0 ? W<float>(0) : (q < 7 ? W<float>(1) : W<float>(7)));
+  else return r;
+}
+
+// Unsigned division by a run-time constant: q = (n * m) >> s, exact for n < 2^31 and d < 2^16.
+struct FastDiv {
+  uint64_t m;
+  uint32_t s, d;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t s = 0;
+  while ((1u << s) < d) ++s;
+  FastDiv f;
+  f.d = d;
+  f.s = 32 + s;
+  f.m = ((uint64_t(1) << f.s) / d) + 1;
+  return f;
+}
+__device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) { return uint32_t((uint64_t(n) * f.m) >> f.s); }
+
+}  // namespace jp

@@ -1,0 +1,181 @@
+"""ctypes binding of the C ABI in include/jp.h (build/libjp.so), for the tests, bench.py and scripts.
+
+The library is the product: there is no Python or CPU fallback. If build/libjp.so is missing, loading
+fails with an error that says how to build it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libjp.so")
+
+JP_OK, JP_ERR_HIP, JP_ERR_ARG, JP_ERR_CAPACITY, JP_ERR_STATE, JP_ERR_NOMEM = range(6)
+PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP64 = 0, 1, 2, 3
+PREC_NAMES = {"fp32": 0, "fp16": 1, "bf16": 2, "fp64": 3}
+MODE_ALL, MODE_SLAB, MODE_SENSOR = 0, 1, 2
+MODE_NAMES = {"all": 0, "slab": 1, "sensor": 2}
+RAW_DTYPE = {PREC_FP32: np.float32, PREC_FP16: np.uint16, PREC_BF16: np.uint16, PREC_FP64: np.float64}
+
+_lib = None
+
+
+class JPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"jp error {code}: {msg}")
+        self.code = code
+
+
+class UniformParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("re", ctypes.c_double), ("u0", ctypes.c_double), ("prec", ctypes.c_int),
+                ("phase", ctypes.c_double * 3), ("scale", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class Derived(ctypes.Structure):
+    _fields_ = [("nu", ctypes.c_double), ("tau0", ctypes.c_double), ("tau1", ctypes.c_double),
+                ("tau2", ctypes.c_double), ("omega0", ctypes.c_double), ("omega1", ctypes.c_double),
+                ("omega2", ctypes.c_double), ("kf", ctypes.c_double), ("nqs01", ctypes.c_double),
+                ("nqi10", ctypes.c_double), ("nqs12", ctypes.c_double), ("nqi21", ctypes.c_double),
+                ("scale", ctypes.c_double * 3), ("prec", ctypes.c_int), ("n", ctypes.c_int),
+                ("cells", ctypes.c_long)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["scale"] = list(self.scale)
+        return d
+
+
+class BenchResult(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_per_step", ctypes.c_double), ("work", ctypes.c_double),
+                ("glups", ctypes.c_double), ("bytes_per_update", ctypes.c_double), ("gbps", ctypes.c_double),
+                ("pool_gb", ctypes.c_double), ("steps", ctypes.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` "
+                               "(or __graft_entry__.build()); there is no fallback path")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, l, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
+        sig = {
+            "jp_version": (ctypes.c_char_p, []),
+            "jp_uniform_create": (i, [ctypes.POINTER(UniformParams), ctypes.POINTER(vp)]),
+            "jp_init": (i, [vp]), "jp_step": (i, [vp, l]), "jp_diag": (i, [vp, vp]),
+            "jp_bench": (i, [vp, l, l, i, ctypes.POINTER(BenchResult)]),
+            "jp_get_derived": (i, [vp, ctypes.POINTER(Derived)]),
+            "jp_get_populations": (i, [vp, i, vp, sz]), "jp_set_populations": (i, [vp, i, vp, sz]),
+            "jp_population_bytes": (i, [vp, i, ctypes.POINTER(sz)]),
+            "jp_sync": (i, [vp]), "jp_last_error": (ctypes.c_char_p, [vp]), "jp_destroy": (None, [vp]),
+            "jp_probe_copy": (i, [i, sz, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header=None):
+    """Function names declared in include/jp.h."""
+    import re
+    header = header or os.path.join(HERE, "..", "include", "jp.h")
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+\**(jp_\w+)\s*\(", txt, re.M)))
+
+
+def probe_copy(nbytes, iters=20, device=0):
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    rc = lib().jp_probe_copy(device, nbytes, iters, ctypes.byref(g), ctypes.byref(ms))
+    if rc:
+        raise JPError(rc, "probe_copy failed")
+    return g.value, ms.value
+
+
+class _Ctx:
+    h = None
+
+    def _check(self, rc):
+        if rc != JP_OK:
+            msg = lib().jp_last_error(self.h).decode() if self.h else ""
+            raise JPError(rc, msg)
+
+    def close(self):
+        if self.h:
+            lib().jp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self):
+        self._check(lib().jp_init(self.h))
+
+    def step(self, n=1):
+        self._check(lib().jp_step(self.h, n))
+
+    def diag(self):
+        out = np.zeros(3)
+        self._check(lib().jp_diag(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def derived(self):
+        d = Derived()
+        self._check(lib().jp_get_derived(self.h, ctypes.byref(d)))
+        return d.as_dict()
+
+    def bench(self, steps, warmup=20, graph=False):
+        r = BenchResult()
+        self._check(lib().jp_bench(self.h, steps, warmup, 1 if graph else 0, ctypes.byref(r)))
+        return r.as_dict()
+
+    def population_bytes(self, level=0):
+        n = ctypes.c_size_t()
+        self._check(lib().jp_population_bytes(self.h, level, ctypes.byref(n)))
+        return n.value
+
+    def raw(self, level=0):
+        nb = self.population_bytes(level)
+        dt = RAW_DTYPE[self.prec]
+        out = np.empty(nb // np.dtype(dt).itemsize, dtype=dt)
+        if nb:
+            self._check(lib().jp_get_populations(self.h, level, out.ctypes.data_as(ctypes.c_void_p), nb))
+        return out
+
+    def set_raw(self, a, level=0):
+        nb = self.population_bytes(level)
+        a = np.ascontiguousarray(a, dtype=RAW_DTYPE[self.prec])
+        self._check(lib().jp_set_populations(self.h, level, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+                    if a.nbytes == nb else JP_ERR_ARG)
+
+    def sync(self):
+        self._check(lib().jp_sync(self.h))
+
+
+class Uniform(_Ctx):
+    """Uniform periodic D3Q19 TGV lattice on the GPU (include/jp.h jp_uniform_create)."""
+
+    def __init__(self, n, re=1600.0, u0=0.05, prec=PREC_FP32, phase=(0.0, 0.0, 0.0), scale=0.0, device=0):
+        self.prec, self.n = prec, n
+        p = UniformParams(n=n, re=re, u0=u0, prec=prec, scale=scale, device=device)
+        for k in range(3):
+            p.phase[k] = phase[k]
+        h = ctypes.c_void_p()
+        rc = lib().jp_uniform_create(ctypes.byref(p), ctypes.byref(h))
+        self.h = h.value
+        self._check(rc)
