@@ -76,9 +76,75 @@ typedef struct {
   long steps;
 } jp_bench_result;
 
+/* Refinement modes of --l1 / --l2 (src/tgv3d_amr.cu:651, 777-797). */
+#define JP_MODE_ALL 0
+#define JP_MODE_SLAB 1
+#define JP_MODE_SENSOR 2
+
+/* 3-level block AMR parameters: the ./amr flags of src/tgv3d_amr.cu:644-656, 674, 685. */
+typedef struct {
+  int n;             /* --n: coarse edge N0 (multiple of 8); the finest level is 4 N0 */
+  double re, u0;     /* --re, --u0 */
+  int prec;          /* JP_PREC_* */
+  double phase[3];   /* TGV phase shift per axis (0 = reference) */
+  int l1_mode;       /* --l1: JP_MODE_* (used at the initial build only, as in the reference) */
+  int l2_mode;       /* --l2 */
+  double frac1;      /* --frac1: target refined fraction of level 0 in (0, 1] */
+  double frac2;      /* --frac2: target refined fraction of the deep level-1 tiles in (0, 1] */
+  int adapt;         /* --adapt: regrid period in coarse steps (used by jp_bench and the driver) */
+  double maxgb;      /* --maxgb: fine-pool budget in GiB, 0 = 75 % of free device memory */
+  int skipprep;      /* --skipprep: profiling only, skips prep + ghost fill (breaks the physics) */
+  int skiprestr;     /* --skiprestr: profiling only, skips restriction (breaks the physics) */
+  double scale[3];   /* FP16/BF16 store scale per level; 0 = default */
+  int device;
+} jp_amr_params;
+
+/* Refinement counts and pool capacities (src/tgv3d_amr.cu:681-683, 708, 735, 752, 759-761). */
+typedef struct {
+  long cnb0, nref0, nf1, nref1, nf2, n_skin0, n_skin1;
+  long cap_nref0, cap1, cap_nref1, cap2, hw_nf1, hw_nf2, ndeep;
+} jp_counts;
+
+/* Table ids for jp_get_table (int32 tables, except the two float sensors). */
+#define JP_TABLE_REF0 0
+#define JP_TABLE_FOF0 1
+#define JP_TABLE_RL0 2
+#define JP_TABLE_L1PCB 3
+#define JP_TABLE_L1OCT 4
+#define JP_TABLE_F1POS 5
+#define JP_TABLE_L1GRID 6
+#define JP_TABLE_REF1 7
+#define JP_TABLE_FOF1 8
+#define JP_TABLE_RL1 9
+#define JP_TABLE_L2PL1 10
+#define JP_TABLE_L2OCT 11
+#define JP_TABLE_SKIN0 12
+#define JP_TABLE_SKIN1 13
+#define JP_TABLE_SEN0 14
+#define JP_TABLE_SEN1 15
+
 /* Creates a uniform context and allocates its two population buffers (src/tgv3d_uniform.cu:157-159).
  * Returns JP_ERR_ARG when n % 8 != 0 or n < 8 (src/tgv3d_uniform.cu:148). */
 int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out);
+
+/* Creates an AMR context: level parameters and pool sizing against the device-memory budget
+ * (src/tgv3d_amr.cu:657-709). The level-2 pool is sized by jp_init, after the first measurement. */
+int jp_amr_create(const jp_amr_params* p, jp_ctx** out);
+
+/* AMR regrid: sensor, exact threshold, marks, scans, block tables, migration, skins, on the device;
+ * only counts come back (src/tgv3d_amr.cu:722-764). JP_ERR_CAPACITY if a count exceeds its pool. */
+int jp_regrid(jp_ctx* c);
+
+/* Test hooks: the initial build / a regrid with the sensors replaced by the given values, so that two
+ * implementations can be fed the same sensor field. sen0 has one value per level-0 tile; sen1_by_pos one
+ * value per level-1 grid position ((2N0/8)^3 entries), looked up through each level-1 tile's position.
+ * Either may be NULL (computed sensor). */
+int jp_init_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
+int jp_regrid_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
+
+int jp_get_counts(jp_ctx* c, jp_counts* out);
+/* Copies table `which` (JP_TABLE_*) to dst (capacity cap elements); *n receives its length. */
+int jp_get_table(jp_ctx* c, int which, void* dst, long cap, long* n);
 
 /* TGV initial state, populations at equilibrium (src/tgv3d_uniform.cu:161 k_init_tgv3d; for AMR also
  * the initial build of src/tgv3d_amr.cu:776-815). */
@@ -87,6 +153,11 @@ int jp_init(jp_ctx* c);
 /* n time steps: uniform k_step + swap (src/tgv3d_uniform.cu:194); AMR: n coarse steps of the nested
  * 1:2:4 subcycle base_step (src/tgv3d_amr.cu:821-837). */
 int jp_step(jp_ctx* c, long n);
+
+/* When on, jp_step replays pairs of steps from a captured hipGraph (recaptured after any call that
+ * changes the refinement or the buffers), as the reference's --graph production loop does
+ * (src/tgv3d_amr.cu:872-881, 896). */
+int jp_use_graph(jp_ctx* c, int on);
 
 /* Energy <|u|^2/2>, enstrophy <|omega|^2/2>, mass <rho>, reduced on the device in FP64 with a fixed
  * order; 24 bytes cross to the host (src/tgv3d_uniform.cu:170-183; src/tgv3d_amr.cu:882-885). */

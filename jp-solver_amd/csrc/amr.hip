@@ -1,0 +1,556 @@
+// AMR kernels on gfx950: fine-level pull steps, coarse-fine interface (prep, ghost fill), restriction,
+// seeding, migration, sensor and the composite macro field (src/tgv3d_amr.cu:69-440, 442-453, 515-536).
+//
+// One workgroup of 512 lanes (8 waves) per 8^3 tile for the per-tile kernels; lanes map to cells in
+// x-fastest order so that each wave touches one contiguous 8x8 plane of a q slice. The fine step looks up
+// the tile's 27 same-level neighbours (precomputed at regrid) once in LDS, then pulls each direction
+// from the own tile, a neighbour tile, or the tile's compact ghost record.
+#include "amr_internal.hpp"
+
+namespace jp {
+
+__device__ inline int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+// Trilinear base and weight of the fine coordinate g on the parent grid: x = (g + 1/2)/2 - 1/2,
+// base = floor(x), t = x - base (src/tgv3d_amr.cu:71-73). Exact in integers: g = 2m gives (m - 1, 3/4),
+// g = 2m + 1 gives (m, 1/4).
+__device__ inline void tri(int g, int& base, float& t) {
+  const int m = g >> 1;
+  if (g & 1) {
+    base = m;
+    t = 0.25f;
+  } else {
+    base = m - 1;
+    t = 0.75f;
+  }
+}
+__device__ inline uint32_t dense_cell(const AmrGeo& G, int ci, int cj, int ck) {
+  return (uint32_t(ck) * G.N0 + cj) * G.N0 + ci;
+}
+__device__ inline uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
+  const int bx = cb % G.cgx, by = (cb / G.cgx) % G.cgx, bz = cb / (G.cgx * G.cgx);
+  return dense_cell(G, bx * MB + (cell & 7), by * MB + ((cell >> 3) & 7), bz * MB + (cell >> 6));
+}
+__device__ inline size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
+
+template <class S> __device__ inline void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
+                                                    typename S::real* out) {
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    out[q] = S::template load<q>(f[tidx(b, q, cell)], dec);
+  });
+}
+template <class S> __device__ inline void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
+                                                     float dec, typename S::real* out) {
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    out[q] = S::template load<q>(f[uint64_t(q) * plane + idx], dec);
+  });
+}
+
+// BGK collision of pulled populations, stored to tile b (src/tgv3d_amr.cu:268-273).
+template <class S> __device__ inline void collide_store(typename S::real* fin, typename S::real om,
+                                                        typename S::raw* __restrict__ fb, int b, int cell, float enc) {
+  using R = typename S::real;
+  const Mom<R> m = moments(fin);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    fb[tidx(b, q, cell)] = S::template store<q>(fin[q] - om * (fin[q] - feq<q>(e)), enc);
+  });
+}
+
+// Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
+// 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
+// to this level); missing sources come from the ghost record.
+template <class S>
+__global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                                   const typename S::real* __restrict__ gh, const int* __restrict__ nb,
+                                                   typename S::real om, float enc, float dec) {
+  using R = typename S::real;
+  __shared__ int snb[kNbr];
+  const int b = blockIdx.x, cell = threadIdx.x;
+  if (cell < kNbr) snb[cell] = nb[size_t(b) * kNbr + cell];
+  __syncthreads();
+  const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
+  const R* __restrict__ g = gh + size_t(b) * kGhostSlots;
+  R fin[Q];
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+    if constexpr (q == 0) {
+      fin[0] = S::template load<0>(fa[tidx(b, 0, cell)], dec);
+    } else {
+      const int si = i - cx, sj = j - cy, sk = k - cz;
+      const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
+      const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
+      const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
+      const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+      if (t >= 0) fin[q] = S::template load<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
+      else fin[q] = g[ghost_slot(q, i, j, k)];
+    }
+  });
+  collide_store<S>(fin, om, fb, b, cell, enc);
+}
+
+// prep = feq + nqs * regularize(f - feq) (src/tgv3d_amr.cu:160-171), listed tiles only.
+template <class S, bool kDense>
+__global__ __launch_bounds__(512) void k_prep(const typename S::raw* __restrict__ f, typename S::real* __restrict__ pr,
+                                              const int* __restrict__ list, AmrGeo G, float nqs, float dec) {
+  using R = typename S::real;
+  const int b = list[blockIdx.x], cell = threadIdx.x;
+  R v[Q], nq[Q];
+  uint32_t di = 0;
+  if constexpr (kDense) {
+    di = dense_of_tile(G, b, cell);
+    load_dense<S>(f, G.g0.plane, di, dec, v);
+  } else {
+    load_tile<S>(f, b, cell, dec, v);
+  }
+  const Mom<R> m = moments(v);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    nq[q] = v[q] - feq<q>(e);
+  });
+  regularize(nq);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const R val = feq<q>(e) + R(nqs) * nq[q];
+    if constexpr (kDense) pr[uint64_t(q) * G.g0.plane + di] = val;
+    else pr[tidx(b, q, cell)] = val;
+  });
+}
+
+// Ghost record of level-1 tiles from prep0 (src/tgv3d_amr.cu:122-132, 173-186): for each slot whose
+// source position lies in an unrefined coarse tile, the trilinear interpolation of that direction.
+template <class R>
+__global__ __launch_bounds__(256) void k_fill_l1(const R* __restrict__ prep0, R* __restrict__ gh1,
+                                                 const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
+                                                 const int* __restrict__ ref0, const uint32_t* __restrict__ slots,
+                                                 AmrGeo G) {
+  const int fb = blockIdx.x;
+  const int cb = l1pcb[fb], oct = l1oct[fb];
+  const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
+  const int ox = cbx * 2 * MB + (oct & 1) * MB, oy = cby * 2 * MB + ((oct >> 1) & 1) * MB, oz = cbz * 2 * MB + (oct >> 2) * MB;
+  for (int s = threadIdx.x; s < kGhostSlots; s += 256) {
+    const uint32_t e = slots[s];
+    const int q = (e >> 9) & 31;
+    const int gfi = pmod(ox + int((e >> 14) & 15) - 1, G.N1), gfj = pmod(oy + int((e >> 18) & 15) - 1, G.N1),
+              gfk = pmod(oz + int((e >> 22) & 15) - 1, G.N1);
+    const int scb = ((gfk / (2 * MB)) * G.cgx + (gfj / (2 * MB))) * G.cgx + (gfi / (2 * MB));
+    if (ref0[scb]) continue;
+    int ib, jb, kb;
+    float tx, ty, tz;
+    tri(gfi, ib, tx);
+    tri(gfj, jb, ty);
+    tri(gfk, kb, tz);
+    const R* __restrict__ p = prep0 + uint64_t(q) * G.g0.plane;
+    R out = R(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+      const uint32_t idx = dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0));
+      out += R(wgt) * p[idx];
+    }
+    gh1[size_t(fb) * kGhostSlots + s] = out;
+  }
+}
+
+// Ghost record of level-2 tiles from prep1 over the scattered level 1, dropped corners renormalised
+// (src/tgv3d_amr.cu:133-144, 187-202).
+template <class R>
+__global__ __launch_bounds__(256) void k_fill_l2(const R* __restrict__ prep1, R* __restrict__ gh2,
+                                                 const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
+                                                 const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                                 const int* __restrict__ ref1, const uint32_t* __restrict__ slots,
+                                                 AmrGeo G) {
+  const int b = blockIdx.x;
+  const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
+  const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
+  const int ox = px * 2 * MB + (oct & 1) * MB, oy = py * 2 * MB + ((oct >> 1) & 1) * MB, oz = pz * 2 * MB + (oct >> 2) * MB;
+  for (int s = threadIdx.x; s < kGhostSlots; s += 256) {
+    const uint32_t e = slots[s];
+    const int q = (e >> 9) & 31;
+    const int g2i = pmod(ox + int((e >> 14) & 15) - 1, G.N2), g2j = pmod(oy + int((e >> 18) & 15) - 1, G.N2),
+              g2k = pmod(oz + int((e >> 22) & 15) - 1, G.N2);
+    const int lbs = l1grid[((g2k / (2 * MB)) * G.l1gx + (g2j / (2 * MB))) * G.l1gx + (g2i / (2 * MB))];
+    if (lbs >= 0 && ref1[lbs]) continue;
+    int ib, jb, kb;
+    float tx, ty, tz, wsum = 0.f;
+    tri(g2i, ib, tx);
+    tri(g2j, jb, ty);
+    tri(g2k, kb, tz);
+    R out = R(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+      const int ci = pmod(ib + di, G.N1), cj = pmod(jb + dj, G.N1), ck = pmod(kb + dk, G.N1);
+      const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
+      if (lb < 0) continue;
+      const int cc = ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7);
+      wsum += wgt;
+      out += R(wgt) * prep1[tidx(lb, q, cc)];
+    }
+    if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
+    gh2[size_t(b) * kGhostSlots + s] = out;
+  }
+}
+
+// Restriction (src/tgv3d_amr.cu:275-298, 358-381): each covered parent cell = mean of its 8 children,
+// then feq + nqi * regularize(mean - feq). One lane per parent cell.
+template <class S, bool kToDense>
+__global__ __launch_bounds__(256) void k_restrict(typename S::raw* __restrict__ cf, const typename S::raw* __restrict__ ff,
+                                                  const int* __restrict__ rl, const int* __restrict__ fof, int nref,
+                                                  AmrGeo G, float nqi, float cenc, float fdec) {
+  using R = typename S::real;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nref * MB3) return;
+  const int r = t / MB3, cell = t % MB3, cb = rl[r];
+  const int lci = cell & 7, lcj = (cell >> 3) & 7, lck = cell >> 6;
+  const int oct = (lck / HALF) * 4 + (lcj / HALF) * 2 + (lci / HALF), fb = fof[cb] + oct;
+  const int fi0 = 2 * (lci % HALF), fj0 = 2 * (lcj % HALF), fk0 = 2 * (lck % HALF);
+  R avg[Q], nq[Q];
+  sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0); });
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+    const int sc = ((fk0 + dk) * MB + (fj0 + dj)) * MB + (fi0 + di);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      avg[q] += R(0.125) * S::template load<q>(ff[tidx(fb, q, sc)], fdec);
+    });
+  }
+  const Mom<R> m = moments(avg);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    nq[q] = avg[q] - feq<q>(e);
+  });
+  regularize(nq);
+  const uint32_t di = kToDense ? dense_of_tile(G, cb, cell) : 0;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
+    if constexpr (kToDense) cf[uint64_t(q) * G.g0.plane + di] = v;
+    else cf[tidx(cb, q, cell)] = v;
+  });
+}
+
+// prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
+// non-equilibrium regularised and rescaled by nqs.
+template <class S> __device__ inline void prolong_l0(const typename S::raw* __restrict__ c0, const AmrGeo& G, int gfi,
+                                                     int gfj, int gfk, float nqs, float dec, typename S::real* out) {
+  using R = typename S::real;
+  int ib, jb, kb;
+  float tx, ty, tz;
+  tri(gfi, ib, tx);
+  tri(gfj, jb, ty);
+  tri(gfk, kb, tz);
+  sfor<Q>([&](auto qc) { out[decltype(qc)::value] = R(0); });
+  for (int c = 0; c < 8; ++c) {
+    const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+    const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+    R fq[Q], nq[Q];
+    load_dense<S>(c0, G.g0.plane, dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0)), dec, fq);
+    const Mom<R> m = moments(fq);
+    const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      nq[q] = fq[q] - feq<q>(e);
+    });
+    regularize(nq);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      out[q] += R(wgt) * (feq<q>(e) + R(nqs) * nq[q]);
+    });
+  }
+}
+// prolong_L1 (src/tgv3d_amr.cu:94-118): parents on the scattered level 1, absent corners dropped and
+// the weights renormalised when 0 < wsum < 0.999.
+template <class S> __device__ inline void prolong_l1(const typename S::raw* __restrict__ f1, const int* __restrict__ l1grid,
+                                                     const AmrGeo& G, int gfi, int gfj, int gfk, float nqs, float dec,
+                                                     typename S::real* out) {
+  using R = typename S::real;
+  int ib, jb, kb;
+  float tx, ty, tz, wsum = 0.f;
+  tri(gfi, ib, tx);
+  tri(gfj, jb, ty);
+  tri(gfk, kb, tz);
+  sfor<Q>([&](auto qc) { out[decltype(qc)::value] = R(0); });
+  for (int c = 0; c < 8; ++c) {
+    const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+    const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+    const int ci = pmod(ib + di, G.N1), cj = pmod(jb + dj, G.N1), ck = pmod(kb + dk, G.N1);
+    const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
+    if (lb < 0) continue;
+    const int cc = ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7);
+    R fq[Q], nq[Q];
+    load_tile<S>(f1, lb, cc, dec, fq);
+    const Mom<R> m = moments(fq);
+    const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      nq[q] = fq[q] - feq<q>(e);
+    });
+    regularize(nq);
+    wsum += wgt;
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      out[q] += R(wgt) * (feq<q>(e) + R(nqs) * nq[q]);
+    });
+  }
+  if (wsum > 0.f && wsum < 0.999f) {
+    const R inv = R(1) / R(wsum);
+    sfor<Q>([&](auto qc) { out[decltype(qc)::value] *= inv; });
+  }
+}
+
+template <class S> __device__ inline void store_tile(typename S::raw* __restrict__ f, int b, int cell,
+                                                     const typename S::real* v, float enc) {
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    f[tidx(b, q, cell)] = S::template store<q>(v[q], enc);
+  });
+}
+
+// Seeding (src/tgv3d_amr.cu:299-308, 382-393) and migration (397-430).
+template <class S>
+__global__ __launch_bounds__(512) void k_seed_l1(const typename S::raw* __restrict__ c0, typename S::raw* __restrict__ f1,
+                                                 const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
+                                                 const int* __restrict__ o_ref0, const int* __restrict__ o_fof0,
+                                                 const typename S::raw* __restrict__ oldf1, AmrGeo G, float nqs,
+                                                 float dec0, float enc1) {
+  const int fb = blockIdx.x, cell = threadIdx.x;
+  const int cb = l1pcb[fb], oct = l1oct[fb];
+  if (o_ref0 && o_ref0[cb]) {  // migration: the parent was refined before, copy its old tile
+    const int ofb = o_fof0[cb] + oct;
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      f1[tidx(fb, q, cell)] = oldf1[tidx(ofb, q, cell)];
+    });
+    return;
+  }
+  const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
+  typename S::real out[Q];
+  prolong_l0<S>(c0, G, cbx * 2 * MB + (oct & 1) * MB + (cell & 7), cby * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7),
+                cbz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs, dec0, out);
+  store_tile<S>(f1, fb, cell, out, enc1);
+}
+template <class S>
+__global__ __launch_bounds__(512) void k_seed_l2(const typename S::raw* __restrict__ f1, typename S::raw* __restrict__ f2,
+                                                 const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
+                                                 const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                                 const int* __restrict__ o_l1grid, const int* __restrict__ o_ref1,
+                                                 const int* __restrict__ o_fof1, const typename S::raw* __restrict__ oldf2,
+                                                 AmrGeo G, float nqs, float dec1, float enc2) {
+  const int b = blockIdx.x, cell = threadIdx.x;
+  const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
+  if (o_l1grid) {  // migration keyed by the level-1 grid position
+    const int ol = o_l1grid[p];
+    if (ol >= 0 && o_ref1[ol]) {
+      const int of2 = o_fof1[ol] + oct;
+      sfor<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f2[tidx(b, q, cell)] = oldf2[tidx(of2, q, cell)];
+      });
+      return;
+    }
+  }
+  const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
+  typename S::real out[Q];
+  prolong_l1<S>(f1, l1grid, G, px * 2 * MB + (oct & 1) * MB + (cell & 7), py * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7),
+                pz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs, dec1, out);
+  store_tile<S>(f2, b, cell, out, enc2);
+}
+
+// Sensor (src/tgv3d_amr.cu:442-453): per tile, max over cells of sqrt(sum_q (f - feq)^2) / rho.
+__device__ inline float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+  return v;
+}
+template <class S, bool kDense>
+__global__ __launch_bounds__(512) void k_sensor(const typename S::raw* __restrict__ f, AmrGeo G, float dec,
+                                                float* __restrict__ sen) {
+  using R = typename S::real;
+  __shared__ float sm[8];
+  const int b = blockIdx.x, cell = threadIdx.x;
+  R v[Q];
+  if constexpr (kDense) load_dense<S>(f, G.g0.plane, dense_of_tile(G, b, cell), dec, v);
+  else load_tile<S>(f, b, cell, dec, v);
+  const Mom<R> m = moments(v);
+  const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
+  R s = R(0);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const R d2 = v[q] - feq<q>(e);
+    s += d2 * d2;
+  });
+  float x = float(sqrt(s) / m.rho);
+  x = wave_max(x);
+  if ((cell & 63) == 0) sm[cell >> 6] = x;
+  __syncthreads();
+  if (cell == 0) {
+    float r = sm[0];
+    for (int w = 1; w < 8; ++w) r = fmaxf(r, sm[w]);
+    sen[b] = r;
+  }
+}
+
+// Composite macro field at 4 N0 (src/tgv3d_amr.cu:515-536): level 2 where present, else level 1 at g/2,
+// else level 0 at g/4.
+template <class S>
+__global__ __launch_bounds__(256) void k_composite(const typename S::raw* __restrict__ c0, const typename S::raw* __restrict__ f1,
+                                                   const typename S::raw* __restrict__ f2, const int* __restrict__ ref1,
+                                                   const int* __restrict__ fof1, const int* __restrict__ l1grid, AmrGeo G,
+                                                   float dec0, float dec1, float dec2, V4<typename S::real>* __restrict__ F) {
+  using R = typename S::real;
+  const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t N2 = G.N2;
+  if (idx >= N2 * N2 * N2) return;
+  const int gi = idx % N2, gj = (idx / N2) % N2, gk = idx / (N2 * N2);
+  const int lb = l1grid[((gk / (2 * MB)) * G.l1gx + (gj / (2 * MB))) * G.l1gx + (gi / (2 * MB))];
+  R v[Q];
+  if (lb >= 0 && ref1[lb]) {
+    const int f2b = fof1[lb] + ((gk % (2 * MB)) / MB) * 4 + ((gj % (2 * MB)) / MB) * 2 + (gi % (2 * MB)) / MB;
+    load_tile<S>(f2, f2b, ((gk % MB) * MB + (gj % MB)) * MB + (gi % MB), dec2, v);
+  } else if (lb >= 0) {
+    const int g1i = gi / 2, g1j = gj / 2, g1k = gk / 2;
+    load_tile<S>(f1, lb, ((g1k % MB) * MB + (g1j % MB)) * MB + (g1i % MB), dec1, v);
+  } else {
+    load_dense<S>(c0, G.g0.plane, dense_cell(G, gi / 4, gj / 4, gk / 4), dec0, v);
+  }
+  const Mom<R> m = moments(v);
+  V4<R> o;
+  o.a = m.rho;
+  o.b = m.ux / m.rho;
+  o.c = m.uy / m.rho;
+  o.d = m.uz / m.rho;
+  F[idx] = o;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------------
+#define JP_DISPATCH(prec, CALL)            \
+  switch (prec) {                          \
+    case 0: { using S = FormF32; CALL; } break;  \
+    case 1: { using S = FormF16; CALL; } break;  \
+    case 2: { using S = FormBF16; CALL; } break; \
+    case 3: { using S = FormF64; CALL; } break;  \
+    default: return hipErrorInvalidValue;  \
+  }
+
+#define RAW(p) static_cast<const typename S::raw*>(p)
+#define WRAW(p) static_cast<typename S::raw*>(p)
+#define REAL(p) static_cast<const typename S::real*>(p)
+#define WREAL(p) static_cast<typename S::real*>(p)
+
+hipError_t amr_fine_step(int prec, int, const void* fa, void* fb, const void* gh, const int* nb, int ntiles, double omega,
+                         float enc, float dec, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step<S>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh), nb,
+                                       typename S::real(omega), enc, dec));
+  return hipGetLastError();
+}
+
+hipError_t amr_prep(int prec, int level, const AmrGeo& G, const void* f, void* prep, const int* list, int n, float nqs,
+                    float dec, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (level == 0) {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_prep<S, true>), dim3(n), dim3(512), 0, st, RAW(f), WREAL(prep), list, G, nqs, dec));
+  } else {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_prep<S, false>), dim3(n), dim3(512), 0, st, RAW(f), WREAL(prep), list, G, nqs, dec));
+  }
+  return hipGetLastError();
+}
+
+hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* prep0, void* gh1, const int* l1pcb, const int* l1oct,
+                       const int* ref0, const uint32_t* slots, int nf1, hipStream_t st) {
+  if (nf1 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l1<typename S::real>), dim3(nf1), dim3(256), 0, st, REAL(prep0), WREAL(gh1),
+                                       l1pcb, l1oct, ref0, slots, G));
+  return hipGetLastError();
+}
+
+hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, const int* ref1, const uint32_t* slots, int nf2,
+                       hipStream_t st) {
+  if (nf2 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l2<typename S::real>), dim3(nf2), dim3(256), 0, st, REAL(prep1), WREAL(gh2),
+                                       l2pl1, l2oct, f1pos, l1grid, ref1, slots, G));
+  return hipGetLastError();
+}
+
+hipError_t amr_restrict(int prec, bool to_dense, const AmrGeo& G, void* cf, const void* ff, const int* rl, const int* fof,
+                        int nref, float nqi, float cenc, float fdec, hipStream_t st) {
+  if (nref <= 0) return hipSuccess;
+  const dim3 grid((nref * MB3 + 255) / 256);
+  if (to_dense) {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_restrict<S, true>), grid, dim3(256), 0, st, WRAW(cf), RAW(ff), rl, fof, nref, G,
+                                         nqi, cenc, fdec));
+  } else {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_restrict<S, false>), grid, dim3(256), 0, st, WRAW(cf), RAW(ff), rl, fof, nref, G,
+                                         nqi, cenc, fdec));
+  }
+  return hipGetLastError();
+}
+
+hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct, int nf1,
+                       float nqs, float dec0, float enc1, hipStream_t st) {
+  if (nf1 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l1<S>), dim3(nf1), dim3(512), 0, st, RAW(c0), WRAW(f1), l1pcb, l1oct,
+                                       nullptr, nullptr, nullptr, G, nqs, dec0, enc1));
+  return hipGetLastError();
+}
+hipError_t amr_migrate_l1(int prec, const AmrGeo& G, const void* oldf1, void* newf1, const void* c0, const int* l1pcb,
+                          const int* l1oct, const int* o_ref0, const int* o_fof0, int nf1, float nqs, float dec0, float enc1,
+                          hipStream_t st) {
+  if (nf1 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l1<S>), dim3(nf1), dim3(512), 0, st, RAW(c0), WRAW(newf1), l1pcb, l1oct,
+                                       o_ref0, o_fof0, RAW(oldf1), G, nqs, dec0, enc1));
+  return hipGetLastError();
+}
+hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, int nf2, float nqs, float dec1, float enc2, hipStream_t st) {
+  if (nf2 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l2<S>), dim3(nf2), dim3(512), 0, st, RAW(f1), WRAW(f2), l2pl1, l2oct, f1pos,
+                                       l1grid, nullptr, nullptr, nullptr, nullptr, G, nqs, dec1, enc2));
+  return hipGetLastError();
+}
+hipError_t amr_migrate_l2(int prec, const AmrGeo& G, const void* oldf2, void* newf2, const void* newf1, const int* l2pl1,
+                          const int* l2oct, const int* f1pos, const int* o_l1grid, const int* o_ref1, const int* o_fof1,
+                          const int* l1grid, int nf2, float nqs, float dec1, float enc2, hipStream_t st) {
+  if (nf2 <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l2<S>), dim3(nf2), dim3(512), 0, st, RAW(newf1), WRAW(newf2), l2pl1, l2oct,
+                                       f1pos, l1grid, o_l1grid, o_ref1, o_fof1, RAW(oldf2), G, nqs, dec1, enc2));
+  return hipGetLastError();
+}
+
+hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const void* f1, const void* f2, const int* ref1,
+                               const int* fof1, const int* l1grid, const float* dec3, void* F, hipStream_t st) {
+  const uint32_t n = uint32_t(G.N2) * G.N2 * G.N2;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_composite<S>), dim3((n + 255) / 256), dim3(256), 0, st, RAW(c0), RAW(f1), RAW(f2),
+                                       ref1, fof1, l1grid, G, dec3[0], dec3[1], dec3[2],
+                                       static_cast<V4<typename S::real>*>(F)));
+  return hipGetLastError();
+}
+
+hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int ntiles, float dec, float* sen, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  if (level == 0) {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_sensor<S, true>), dim3(ntiles), dim3(512), 0, st, RAW(f), G, dec, sen));
+  } else {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_sensor<S, false>), dim3(ntiles), dim3(512), 0, st, RAW(f), G, dec, sen));
+  }
+  return hipGetLastError();
+}
+
+}  // namespace jp

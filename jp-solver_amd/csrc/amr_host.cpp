@@ -1,7 +1,430 @@
-// AMR orchestration: placeholder until the AMR kernels land.
+// AMR orchestration (host side of the C ABI): pool sizing, the initial build, the regrid pipeline and
+// the nested 1:2:4 subcycle (src/tgv3d_amr.cu:657-837). All data stays on the device; the host reads
+// only the refinement counts it needs to size launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "amr_internal.hpp"
 #include "ctx.hpp"
+
 namespace jp {
-int amr_init(jp_ctx* c) { return fail(c, JP_ERR_STATE, "AMR not built"); }
-int amr_base_step(jp_ctx* c, hipStream_t) { return fail(c, JP_ERR_STATE, "AMR not built"); }
-int amr_composite(jp_ctx* c, void*) { return fail(c, JP_ERR_STATE, "AMR not built"); }
+
+static AmrGeo geo(const jp_ctx* c) {
+  AmrGeo G;
+  G.g0 = c->g0;
+  G.N0 = c->amr.N;
+  G.cgx = c->amr.cgx;
+  G.l1gx = c->amr.l1gx;
+  G.N1 = c->amr.N1;
+  G.N2 = c->amr.N2;
+  return G;
+}
+static size_t real_bytes(int prec) { return prec == JP_PREC_FP64 ? 8 : 4; }
+static float enc(const jp_ctx* c, int l) { return float(c->d.scale[l]); }
+static float dec(const jp_ctx* c, int l) { return float(1.0 / c->d.scale[l]); }
+
+// fill table of the compact ghost record: one entry per slot
+static std::vector<uint32_t> ghost_fill_table() {
+  std::vector<uint32_t> t(kGhostSlots, 0xffffffffu);
+  for (int q = 1; q < Q; ++q)
+    for (int cell = 0; cell < MB3; ++cell) {
+      const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
+      const int si = i - CX(q), sj = j - CY(q), sk = k - CZ(q);
+      if (si >= 0 && si < 8 && sj >= 0 && sj < 8 && sk >= 0 && sk < 8) continue;
+      const int s = ghost_slot(q, i, j, k);
+      t[s] = uint32_t(cell) | uint32_t(q) << 9 | uint32_t(si + 1) << 14 | uint32_t(sj + 1) << 18 | uint32_t(sk + 1) << 22;
+    }
+  return t;
+}
+
+// ---- counts ----
+static int read_counts(jp_ctx* c) {
+  HCK(c, hipMemcpyAsync(c->amr.hcnt, c->amr.dcnt, sizeof(int) * CNT_N, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+static int fill(jp_ctx* c, int* p, int v, size_t n) {
+  HCK(c, fill_i32(p, v, n, c->st));
+  return JP_OK;
+}
+
+static bool pool_fits(const AmrState& a, long c1, long c2) {
+  // src/tgv3d_amr.cu:687-688: (f x2 + ghost box) per fine tile in FP32, +20 % for tables
+  const double PB = double(Q) * 4 * (2 * MB3 + 1000);
+  return double(c1 + c2) * PB * 6 / 5 < a.budget;
+}
+
+static int alloc_l2_pool(jp_ctx* c) {
+  AmrState& a = c->amr;
+  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec);
+  const size_t tile = size_t(Q) * MB3;
+  if (int rc = ctx_alloc(c, &a.f2a, size_t(a.cap2) * tile * e, "f2a")) return rc;
+  if (int rc = ctx_alloc(c, &a.f2b, size_t(a.cap2) * tile * e, "f2b")) return rc;
+  if (int rc = ctx_alloc(c, &a.gh2, size_t(a.cap2) * kGhostSlots * r, "gh2")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.l2pl1, a.cap2, "l2pl1")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.l2oct, a.cap2, "l2oct")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.nb2, size_t(a.cap2) * kNbr, "nb2")) return rc;
+  return JP_OK;
+}
+
+// skins (src/tgv3d_amr.cu:758-762) and the 27-neighbour tables of both fine levels
+static int finish_tables(jp_ctx* c) {
+  AmrState& a = c->amr;
+  HCK(c, rg_dilate_l0(a.ref0, a.cnb0, a.cgx, a.flags, c->st));
+  HCK(c, rg_compact(a.flags, a.cnb0, a.skin0, a.dcnt, CNT_SKIN0, c->st));
+  if (a.nf1) {
+    HCK(c, rg_dilate_l1(a.ref1, a.f1pos, a.l1grid, a.nf1, a.l1gx, a.flags, c->st));
+    HCK(c, rg_compact(a.flags, a.nf1, a.skin1, a.dcnt, CNT_SKIN1, c->st));
+    HCK(c, rg_nbr_l1(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.nb1, c->st));
+  }
+  if (a.nf2) HCK(c, rg_nbr_l2(a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.fof1, a.l1gx, a.nf2, a.nb2, c->st));
+  if (int r = read_counts(c)) return r;
+  a.n_skin0 = a.hcnt[CNT_SKIN0];
+  a.n_skin1 = a.nf1 ? a.hcnt[CNT_SKIN1] : 0;
+  return JP_OK;
+}
+
+// level-1 marks -> scan -> counts -> tables (common to init and regrid). ref0 is already marked.
+static int build_l1_tables(jp_ctx* c) {
+  AmrState& a = c->amr;
+  HCK(c, rg_scan(a.ref0, a.cnb0, a.rank, a.dcnt, CNT_NREF0, c->st));
+  if (int r = read_counts(c)) return r;
+  a.nref0 = a.hcnt[CNT_NREF0];
+  a.nf1 = 8 * a.nref0;
+  a.hw_nf1 = std::max(a.hw_nf1, long(a.nf1));
+  if (a.nf1 > a.cap1)
+    return fail(c, JP_ERR_CAPACITY, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
+                                        std::to_string(a.cap1) + " (threshold ties); raise --maxgb or lower --frac1");
+  if (int r = fill(c, a.l1grid, -1, size_t(a.NG1))) return r;
+  if (int r = fill(c, a.fof0, -1, size_t(a.cnb0))) return r;
+  if (a.nref0)
+    HCK(c, rg_scatter_l1(a.ref0, a.rank, a.cnb0, a.cgx, a.l1gx, a.cap1, a.fof0, a.rl0, a.l1pcb, a.l1oct, a.l1grid, a.f1pos,
+                         a.dcnt, c->st));
+  return JP_OK;
+}
+
+static int scan_l2(jp_ctx* c) {
+  AmrState& a = c->amr;
+  HCK(c, rg_scan(a.ref1, a.nf1, a.rank, a.dcnt, CNT_NREF1, c->st));
+  if (int r = read_counts(c)) return r;
+  a.nref1 = a.hcnt[CNT_NREF1];
+  a.nf2 = 8 * a.nref1;
+  a.hw_nf2 = std::max(a.hw_nf2, long(a.nf2));
+  return JP_OK;
+}
+
+static int upload_sensor(jp_ctx* c, const float* sen0, int n) {
+  HCK(c, hipMemcpyAsync(c->amr.sen0, sen0, size_t(n) * 4, hipMemcpyHostToDevice, c->st));
+  return JP_OK;
+}
+static int sensor_l1(jp_ctx* c, const float* sen1_by_pos) {
+  AmrState& a = c->amr;
+  HCK(c, amr_sensor(c->prec, 1, geo(c), a.f1a, a.nf1, dec(c, 1), a.sen1, c->st));
+  if (sen1_by_pos) {
+    HCK(c, hipMemcpyAsync(a.sen1pos, sen1_by_pos, size_t(a.NG1) * 4, hipMemcpyHostToDevice, c->st));
+    HCK(c, rg_gather_by_pos(a.sen1pos, a.f1pos, a.nf1, a.sen1, c->st));
+  }
+  HCK(c, rg_deepmask(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.deep, c->st));
+  return JP_OK;
+}
+
+// ---- initial build, src/tgv3d_amr.cu:776-815 ----
+int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
+  AmrState& a = c->amr;
+  const AmrGeo G = geo(c);
+  drop_graph(c);
+  HCK(c, dense_init(c->prec, c->c0a, c->g0, c->u0, c->d.kf, c->phase, enc(c, 0), c->st));
+  if (int r = fill(c, a.dcnt, 0, CNT_N)) return r;
+  a.hw_nf1 = a.hw_nf2 = 0;
+  if (a.l1_mode == JP_MODE_ALL) HCK(c, rg_set(a.ref0, a.cnb0, 0, a.cgx, nullptr, c->st));
+  else if (a.l1_mode == JP_MODE_SLAB) HCK(c, rg_set(a.ref0, a.cnb0, 1, a.cgx, nullptr, c->st));
+  else {
+    for (int s = 0; s < 300; ++s) {  // develop(300): unrecorded coarse steps before the first marking
+      HCK(c, dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, c->st));
+      std::swap(c->c0a, c->c0b);
+    }
+    HCK(c, amr_sensor(c->prec, 0, G, c->c0a, a.cnb0, dec(c, 0), a.sen0, c->st));
+    if (sen0)
+      if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
+    HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
+    HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
+  }
+  if (int r = build_l1_tables(c)) return r;
+  if (a.nf1) HCK(c, amr_seed_l1(c->prec, G, c->c0a, a.f1a, a.l1pcb, a.l1oct, a.nf1, float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
+  a.nref1 = a.nf2 = 0;
+  if (a.nf1) {
+    if (int r = sensor_l1(c, sen1_by_pos)) return r;
+    if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
+    if (a.l2_mode == JP_MODE_ALL) HCK(c, rg_set(a.ref1, a.nf1, 2, a.cgx, a.deep, c->st));
+    else {
+      // init does not cap the level-2 target (src/tgv3d_amr.cu:795)
+      HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, INT_MAX, a.thr, a.dcnt, CNT_NDEEP, c->st));
+      HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
+    }
+    if (int r = scan_l2(c)) return r;
+    a.cap_nref1 = std::max(1, int(std::ceil(a.nref1 * 2.5)));
+    if (a.l2_mode == JP_MODE_ALL) a.cap_nref1 = std::max(a.cap_nref1, a.nf1);
+    a.cap_nref1 = std::min(a.cap_nref1, a.cap1);
+    while (a.cap_nref1 > 1 && !pool_fits(a, a.cap1, 8L * a.cap_nref1)) a.cap_nref1--;
+    a.cap2 = 8 * a.cap_nref1;
+  } else {
+    a.cap_nref1 = 1;
+    a.cap2 = 8;
+  }
+  if (!a.f2a)
+    if (int r = alloc_l2_pool(c)) return r;
+  if (a.nf2 > a.cap2)
+    return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
+                                        std::to_string(a.cap2) + "; raise --maxgb");
+  if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
+  if (a.nf2) {
+    HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
+    HCK(c, amr_seed_l2(c->prec, G, a.f1a, a.f2a, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1),
+                       enc(c, 2), c->st));
+  }
+  if (int r = finish_tables(c)) return r;
+  if (a.hcnt[CNT_OVERFLOW]) return fail(c, JP_ERR_CAPACITY, "pool capacity exceeded while scattering tables");
+  c->initialised = true;
+  return JP_OK;
+}
+int amr_init(jp_ctx* c) { return amr_init_with(c, nullptr, nullptr); }
+
+// ---- regrid, src/tgv3d_amr.cu:722-764 ----
+int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
+  AmrState& a = c->amr;
+  const AmrGeo G = geo(c);
+  drop_graph(c);
+  HCK(c, rg_copy_i32(a.ref0, a.o_ref0, size_t(a.cnb0), c->st));
+  HCK(c, rg_copy_i32(a.fof0, a.o_fof0, size_t(a.cnb0), c->st));
+  HCK(c, rg_copy_i32(a.l1grid, a.o_l1grid, size_t(a.NG1), c->st));
+  HCK(c, rg_copy_i32(a.ref1, a.o_ref1, size_t(a.cap1), c->st));
+  HCK(c, rg_copy_i32(a.fof1, a.o_fof1, size_t(a.cap1), c->st));
+  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, a.cnb0, dec(c, 0), a.sen0, c->st));
+  if (sen0)
+    if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
+  HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
+  HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
+  if (int r = build_l1_tables(c)) return r;
+  if (a.nf1) {
+    HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, a.nf1, float(c->d.nqs01),
+                          dec(c, 0), enc(c, 1), c->st));
+    std::swap(a.f1a, a.f1b);
+  }
+  a.nref1 = a.nf2 = 0;
+  if (a.nf1) {
+    if (int r = sensor_l1(c, sen1_by_pos)) return r;
+    HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, a.cap_nref1, a.thr, a.dcnt, CNT_NDEEP, c->st));
+    if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
+    HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
+    if (int r = scan_l2(c)) return r;
+    if (a.nf2 > a.cap2)
+      return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
+                                          std::to_string(a.cap2) + " (threshold ties); raise --maxgb or lower --frac2");
+    if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
+    if (a.nref1) HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
+    if (a.nf2) {
+      HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
+                            a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
+      std::swap(a.f2a, a.f2b);
+    }
+  }
+  if (int r = finish_tables(c)) return r;
+  if (a.hcnt[CNT_OVERFLOW]) return fail(c, JP_ERR_CAPACITY, "pool capacity exceeded while scattering tables");
+  return JP_OK;
+}
+
+// ---- one coarse step of the nested 1:2:4 subcycle, src/tgv3d_amr.cu:821-837 ----
+int amr_base_step(jp_ctx* c, hipStream_t st) {
+  AmrState& a = c->amr;
+  const AmrGeo G = geo(c);
+  const int p = c->prec;
+  HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
+  std::swap(c->c0a, c->c0b);
+  if (!a.nf1) return JP_OK;
+  if (!a.skipprep) {
+    HCK(c, amr_prep(p, 0, G, c->c0a, a.prep0, a.skin0, a.n_skin0, float(c->d.nqs01), dec(c, 0), st));
+    HCK(c, amr_fill_l1(p, G, a.prep0, a.gh1, a.l1pcb, a.l1oct, a.ref0, a.slots, a.nf1, st));
+  }
+  for (int s1 = 0; s1 < 2; ++s1) {
+    if (a.nf2) {
+      if (!a.skipprep) {
+        HCK(c, amr_prep(p, 1, G, a.f1a, a.prep1, a.skin1, a.n_skin1, float(c->d.nqs12), dec(c, 1), st));
+        HCK(c, amr_fill_l2(p, G, a.prep1, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.slots, a.nf2, st));
+      }
+      for (int s2 = 0; s2 < 2; ++s2) {
+        HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+        std::swap(a.f2a, a.f2b);
+      }
+      if (!a.skiprestr)
+        HCK(c, amr_restrict(p, false, G, a.f1a, a.f2a, a.rl1, a.fof1, a.nref1, float(c->d.nqi21), enc(c, 1), dec(c, 2), st));
+    }
+    HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+    std::swap(a.f1a, a.f1b);
+  }
+  if (!a.skiprestr)
+    HCK(c, amr_restrict(p, true, G, c->c0a, a.f1a, a.rl0, a.fof0, a.nref0, float(c->d.nqi10), enc(c, 0), dec(c, 1), st));
+  return JP_OK;
+}
+
+int amr_composite(jp_ctx* c, void* field) {
+  AmrState& a = c->amr;
+  const float d3[3] = {dec(c, 0), dec(c, 1), dec(c, 2)};
+  HCK(c, amr_composite_field(c->prec, geo(c), c->c0a, a.f1a, a.f2a, a.ref1, a.fof1, a.l1grid, d3, field, c->st));
+  return JP_OK;
+}
+
+// Creates the context: level parameters (src/tgv3d_amr.cu:658-664) and pools (675-709).
+int amr_create(const jp_amr_params* p, jp_ctx* c) {
+  AmrState& a = c->amr;
+  const int N = p->n;
+  a.N = N;
+  a.cgx = N / MB;
+  a.cnb0 = a.cgx * a.cgx * a.cgx;
+  a.l1gx = 2 * a.cgx;
+  a.N1 = 2 * N;
+  a.N2 = 4 * N;
+  a.NG1 = a.l1gx * a.l1gx * a.l1gx;
+  a.l1_mode = p->l1_mode;
+  a.l2_mode = p->l2_mode;
+  a.frac1 = p->frac1;
+  a.frac2 = p->frac2;
+  a.adapt = p->adapt;
+  a.skipprep = p->skipprep;
+  a.skiprestr = p->skiprestr;
+  jp_derived& d = c->d;
+  d.n = N;
+  d.prec = p->prec;
+  d.nu = p->u0 * double(N) / (2.0 * M_PI * p->re);
+  d.tau0 = 3.0 * d.nu + 0.5;
+  d.tau1 = 2.0 * d.tau0 - 0.5;
+  d.tau2 = 2.0 * d.tau1 - 0.5;
+  const bool f64 = p->prec == JP_PREC_FP64;
+  d.omega0 = f64 ? 1.0 / d.tau0 : double(float(1.0 / d.tau0));
+  d.omega1 = f64 ? 1.0 / d.tau1 : double(float(1.0 / d.tau1));
+  d.omega2 = f64 ? 1.0 / d.tau2 : double(float(1.0 / d.tau2));
+  d.nqs01 = float((d.tau1 - 1.0) / (2.0 * (d.tau0 - 1.0)));
+  d.nqi10 = float((2.0 * (d.tau0 - 1.0)) / (d.tau1 - 1.0));
+  d.nqs12 = float((d.tau2 - 1.0) / (2.0 * (d.tau1 - 1.0)));
+  d.nqi21 = float((2.0 * (d.tau1 - 1.0)) / (d.tau2 - 1.0));
+  d.kf = 2.0 * M_PI / double(N);
+  d.cells = long(N) * N * N;
+  const bool shifted = p->prec == JP_PREC_FP16 || p->prec == JP_PREC_BF16;
+  for (int l = 0; l < 3; ++l) {
+    const double s = shifted ? (p->scale[l] > 0 ? p->scale[l] : default_scale(p->u0)) : 1.0;
+    if (s != std::exp2(std::round(std::log2(s)))) return fail(c, JP_ERR_ARG, "scale must be a power of two");
+    d.scale[l] = s;
+  }
+  size_t freeB = 0, totB = 0;
+  HCK(c, hipMemGetInfo(&freeB, &totB));
+  a.budget = p->maxgb > 0 ? p->maxgb * double(1ull << 30) : double(freeB) * 0.75;
+  a.cap_nref0 = (a.l1_mode == JP_MODE_ALL) ? a.cnb0 : std::min(a.cnb0, std::max(1, int(std::ceil(a.frac1 * a.cnb0 * 1.4))));
+  while (a.cap_nref0 > 1 && !pool_fits(a, 8L * a.cap_nref0, 0)) a.cap_nref0--;
+  a.cap1 = 8 * a.cap_nref0;
+  a.cap_nref1 = INT_MAX;
+  // pools
+  c->g0 = make_dense(N, 0);
+  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
+  const size_t dense_b = size_t(Q) * c->g0.plane;
+  if (int rc = ctx_alloc(c, &c->c0a, dense_b * e, "c0a")) return rc;
+  if (int rc = ctx_alloc(c, &c->c0b, dense_b * e, "c0b")) return rc;
+  if (int rc = ctx_alloc(c, &a.prep0, dense_b * r, "prep0")) return rc;
+  if (int rc = ctx_alloc(c, &a.f1a, size_t(a.cap1) * tile * e, "f1a")) return rc;
+  if (int rc = ctx_alloc(c, &a.f1b, size_t(a.cap1) * tile * e, "f1b")) return rc;
+  if (int rc = ctx_alloc(c, &a.prep1, size_t(a.cap1) * tile * r, "prep1")) return rc;
+  if (int rc = ctx_alloc(c, &a.gh1, size_t(a.cap1) * kGhostSlots * r, "gh1")) return rc;
+  const size_t n01 = size_t(std::max(a.cnb0, a.cap1));
+  struct T {
+    int** p;
+    size_t n;
+    const char* w;
+  } tabs[] = {{&a.ref0, size_t(a.cnb0), "ref0"},     {&a.fof0, size_t(a.cnb0), "fof0"},     {&a.o_ref0, size_t(a.cnb0), "o_ref0"},
+              {&a.o_fof0, size_t(a.cnb0), "o_fof0"}, {&a.skin0, size_t(a.cnb0), "skin0"},   {&a.l1pcb, size_t(a.cap1), "l1pcb"},
+              {&a.l1oct, size_t(a.cap1), "l1oct"},   {&a.rl0, size_t(a.cap1), "rl0"},       {&a.f1pos, size_t(a.cap1), "f1pos"},
+              {&a.ref1, size_t(a.cap1), "ref1"},     {&a.fof1, size_t(a.cap1), "fof1"},     {&a.o_ref1, size_t(a.cap1), "o_ref1"},
+              {&a.o_fof1, size_t(a.cap1), "o_fof1"}, {&a.rl1, size_t(a.cap1), "rl1"},       {&a.deep, size_t(a.cap1), "deep"},
+              {&a.skin1, size_t(a.cap1), "skin1"},   {&a.l1grid, size_t(a.NG1), "l1grid"},  {&a.o_l1grid, size_t(a.NG1), "o_l1grid"},
+              {&a.rank, n01, "rank"},                {&a.flags, n01, "flags"},               {&a.nb1, size_t(a.cap1) * kNbr, "nb1"},
+              {&a.dcnt, size_t(CNT_N), "counts"}};
+  for (auto& t : tabs)
+    if (int rc = ctx_alloc_t(c, t.p, t.n, t.w)) return rc;
+  if (int rc = ctx_alloc_t(c, &a.sen0, size_t(a.cnb0), "sen0")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.sen1, size_t(a.cap1), "sen1")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.sen1pos, size_t(a.NG1), "sen1 by position")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.thr, 4, "threshold")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.slots, size_t(kGhostSlots), "ghost fill table")) return rc;
+  const std::vector<uint32_t> slots = ghost_fill_table();
+  HCK(c, hipMemcpy(a.slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice));
+  HCK(c, hipHostMalloc(reinterpret_cast<void**>(&a.hcnt), sizeof(int) * CNT_N, hipHostMallocDefault));
+  std::memset(a.hcnt, 0, sizeof(int) * CNT_N);
+  // tables start empty
+  if (int rc = fill(c, a.ref1, 0, size_t(a.cap1))) return rc;
+  if (int rc = fill(c, a.fof1, -1, size_t(a.cap1))) return rc;
+  if (int rc = fill(c, a.l1grid, -1, size_t(a.NG1))) return rc;
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+
+int amr_counts(jp_ctx* c, jp_counts* o) {
+  const AmrState& a = c->amr;
+  o->cnb0 = a.cnb0;
+  o->nref0 = a.nref0;
+  o->nf1 = a.nf1;
+  o->nref1 = a.nref1;
+  o->nf2 = a.nf2;
+  o->n_skin0 = a.n_skin0;
+  o->n_skin1 = a.n_skin1;
+  o->cap_nref0 = a.cap_nref0;
+  o->cap1 = a.cap1;
+  o->cap_nref1 = a.cap_nref1 == INT_MAX ? -1 : a.cap_nref1;
+  o->cap2 = a.cap2;
+  o->hw_nf1 = a.hw_nf1;
+  o->hw_nf2 = a.hw_nf2;
+  o->ndeep = a.hcnt ? a.hcnt[CNT_NDEEP] : 0;
+  return JP_OK;
+}
+
+int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n) {
+  const AmrState& a = c->amr;
+  const void* src = nullptr;
+  long len = 0;
+  switch (which) {
+    case JP_TABLE_REF0: src = a.ref0; len = a.cnb0; break;
+    case JP_TABLE_FOF0: src = a.fof0; len = a.cnb0; break;
+    case JP_TABLE_RL0: src = a.rl0; len = a.nref0; break;
+    case JP_TABLE_L1PCB: src = a.l1pcb; len = a.nf1; break;
+    case JP_TABLE_L1OCT: src = a.l1oct; len = a.nf1; break;
+    case JP_TABLE_F1POS: src = a.f1pos; len = a.nf1; break;
+    case JP_TABLE_L1GRID: src = a.l1grid; len = a.NG1; break;
+    case JP_TABLE_REF1: src = a.ref1; len = a.nf1; break;
+    case JP_TABLE_FOF1: src = a.fof1; len = a.nf1; break;
+    case JP_TABLE_RL1: src = a.rl1; len = a.nref1; break;
+    case JP_TABLE_L2PL1: src = a.l2pl1; len = a.nf2; break;
+    case JP_TABLE_L2OCT: src = a.l2oct; len = a.nf2; break;
+    case JP_TABLE_SKIN0: src = a.skin0; len = a.n_skin0; break;
+    case JP_TABLE_SKIN1: src = a.skin1; len = a.n_skin1; break;
+    case JP_TABLE_SEN0: src = a.sen0; len = a.cnb0; break;
+    case JP_TABLE_SEN1: src = a.sen1; len = a.nf1; break;
+    default: return fail(c, JP_ERR_ARG, "unknown table id");
+  }
+  if (n) *n = len;
+  if (len > cap) return fail(c, JP_ERR_ARG, "table buffer too small");
+  if (len > 0) {
+    HCK(c, hipMemcpyAsync(dst, src, size_t(len) * 4, hipMemcpyDeviceToHost, c->st));
+    HCK(c, hipStreamSynchronize(c->st));
+  }
+  return JP_OK;
+}
+
+// Work of one coarse step in cell updates, fine levels weighted by their subcycles (src/tgv3d_amr.cu:843).
+double amr_work(const jp_ctx* c) { return double(c->amr.cnb0 + 2L * c->amr.nf1 + 4L * c->amr.nf2) * MB3; }
+bool amr_dynamic(const jp_ctx* c) {
+  return c->amr.l1_mode == JP_MODE_SENSOR || c->amr.l2_mode == JP_MODE_SENSOR;
+}
+
 }  // namespace jp

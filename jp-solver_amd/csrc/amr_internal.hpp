@@ -1,0 +1,107 @@
+// Launch interface of the AMR kernels (amr.hip, regrid.hip) used by the orchestration in amr_host.cpp.
+//
+// Level 0 is the dense lattice (planes, see internal.hpp). Levels 1 and 2 are scattered tiles of 8^3
+// cells stored tile-major. Each fine tile keeps a compact ghost record of the 1824 (q, cell) pairs whose
+// pull source lies outside the tile (64 per axis direction, 120 per diagonal direction), instead of the
+// reference's 19 x 1000 ghost box (src/tgv3d_amr.cu:179-185): the step reads exactly those entries.
+#pragma once
+#include "internal.hpp"
+
+namespace jp {
+
+constexpr int kGhostSlots = 1824;
+constexpr int kNbr = 27;
+
+// Fill-table entry of one ghost slot: cell | q << 9 | (si+1) << 14 | (sj+1) << 18 | (sk+1) << 22, where
+// (si, sj, sk) in [-1, 8] is the source position relative to the tile.
+// Slot of (q, destination cell) in a tile's ghost record; valid when the source cell - c_q is outside.
+__host__ __device__ inline int ghost_base(int q) { return q < 7 ? (q - 1) * 64 : 384 + (q - 7) * 120; }
+__host__ __device__ inline int ghost_slot(int q, int i, int j, int k) {
+  const int cx = CX(q), cy = CY(q), cz = CZ(q);
+  const int nz = (cx != 0) + (cy != 0) + (cz != 0);
+  if (nz == 1) {
+    if (cx != 0) return ghost_base(q) + k * 8 + j;
+    if (cy != 0) return ghost_base(q) + k * 8 + i;
+    return ghost_base(q) + j * 8 + i;
+  }
+  int ia, ib, it, ca;
+  if (cz == 0) {
+    ia = i; ib = j; it = k; ca = cx;
+  } else if (cy == 0) {
+    ia = i; ib = k; it = j; ca = cx;
+  } else {
+    ia = j; ib = k; it = i; ca = cy;
+  }
+  const int ea = ca > 0 ? 0 : 7;
+  const int r = (ia == ea) ? ib : 8 + (ca > 0 ? ia - 1 : ia);
+  return ghost_base(q) + it * 15 + r;
+}
+
+// Geometry shared by the AMR kernels.
+struct AmrGeo {
+  Dense g0;       // level-0 planes
+  int N0, cgx, l1gx, N1, N2;
+};
+
+// ---- amr.hip ----
+hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const void* gh, const int* nb, int ntiles,
+                         double omega, float enc, float dec, hipStream_t st);
+// prep = feq + nqs * regularize(f - feq) over the listed tiles; level 0 reads/writes dense planes
+hipError_t amr_prep(int prec, int level, const AmrGeo& G, const void* f, void* prep, const int* list, int n,
+                    float nqs, float dec, hipStream_t st);
+hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* prep0, void* gh1, const int* l1pcb, const int* l1oct,
+                       const int* ref0, const uint32_t* slots, int nf1, hipStream_t st);
+hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, const int* ref1, const uint32_t* slots, int nf2,
+                       hipStream_t st);
+// level 1 -> 0 (coarse dense) when to_dense, else level 2 -> 1
+hipError_t amr_restrict(int prec, bool to_dense, const AmrGeo& G, void* cf, const void* ff, const int* rl,
+                        const int* fof, int nref, float nqi, float cenc, float fdec, hipStream_t st);
+hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct,
+                       int nf1, float nqs, float dec0, float enc1, hipStream_t st);
+hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, int nf2, float nqs, float dec1, float enc2,
+                       hipStream_t st);
+hipError_t amr_migrate_l1(int prec, const AmrGeo& G, const void* oldf1, void* newf1, const void* c0,
+                          const int* l1pcb, const int* l1oct, const int* o_ref0, const int* o_fof0, int nf1, float nqs,
+                          float dec0, float enc1, hipStream_t st);
+hipError_t amr_migrate_l2(int prec, const AmrGeo& G, const void* oldf2, void* newf2, const void* newf1,
+                          const int* l2pl1, const int* l2oct, const int* f1pos, const int* o_l1grid, const int* o_ref1,
+                          const int* o_fof1, const int* l1grid, int nf2, float nqs, float dec1, float enc2,
+                          hipStream_t st);
+hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const void* f1, const void* f2,
+                               const int* ref1, const int* fof1, const int* l1grid, const float* dec3, void* F,
+                               hipStream_t st);
+hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int ntiles, float dec, float* sen,
+                      hipStream_t st);
+
+// ---- regrid.hip ----
+// Device counters (int slots of AmrState::dcnt).
+enum { CNT_NREF0 = 0, CNT_NREF1, CNT_SKIN0, CNT_SKIN1, CNT_NDEEP, CNT_OVERFLOW, CNT_TGT, CNT_NSEL0, CNT_N };
+// thr = ascending-sorted[count - tgt] of sen (entries with mask != 0 when mask is given), tgt =
+// min(cap, max(1, int(frac * count))); writes count to cnt[count_slot] and thr to *thr.
+hipError_t rg_select(const float* sen, const int* mask, int n, double frac, int cap, float* thr, int* cnt,
+                     int count_slot, hipStream_t st);
+hipError_t rg_mark_thr(const float* sen, const float* thr, int* ref, int n, hipStream_t st);
+hipError_t rg_mark_ref1(const float* sen, const float* thr, const int* deep, const int* cnt, int* ref1, int nf1,
+                        hipStream_t st);
+hipError_t rg_set(int* ref, int n, int mode, int cgx, const int* deep, hipStream_t st);  // 0 all, 1 slab, 2 deep
+hipError_t rg_deepmask(const int* f1pos, const int* l1grid, int l1gx, int nf1, int* deep, hipStream_t st);
+// exclusive scan of 0/1 flags into rank; total into cnt[slot]
+hipError_t rg_scan(const int* flags, int n, int* rank, int* cnt, int slot, hipStream_t st);
+// ascending indices i with flags[i] != 0 into out; count into cnt[slot]
+hipError_t rg_compact(const int* flags, int n, int* out, int* cnt, int slot, hipStream_t st);
+hipError_t rg_scatter_l1(const int* ref, const int* rank, int cnb0, int cgx, int l1gx, int cap1, int* fof0, int* rl0,
+                         int* l1pcb, int* l1oct, int* l1grid, int* f1pos, int* cnt, hipStream_t st);
+hipError_t rg_scatter_l2(const int* ref1, const int* rank, int nf1, int cap2, int* fof1, int* rl1, int* l2pl1,
+                         int* l2oct, int* cnt, hipStream_t st);
+hipError_t rg_dilate_l0(const int* ref0, int cnb0, int cgx, int* flags, hipStream_t st);
+hipError_t rg_dilate_l1(const int* ref1, const int* f1pos, const int* l1grid, int nf1, int l1gx, int* flags,
+                        hipStream_t st);
+hipError_t rg_nbr_l1(const int* f1pos, const int* l1grid, int l1gx, int nf1, int* nb1, hipStream_t st);
+hipError_t rg_nbr_l2(const int* l2pl1, const int* l2oct, const int* f1pos, const int* l1grid, const int* ref1,
+                     const int* fof1, int l1gx, int nf2, int* nb2, hipStream_t st);
+hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, float* out, hipStream_t st);
+hipError_t rg_copy_i32(const int* src, int* dst, size_t n, hipStream_t st);
+
+}  // namespace jp

@@ -130,6 +130,59 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
   return JP_OK;
 }
 
+int jp_amr_create(const jp_amr_params* p, jp_ctx** out) {
+  if (!p || !out) return JP_ERR_ARG;
+  auto* c = new jp_ctx();
+  *out = c;
+  c->kind = 1;
+  if (p->n < 8 || p->n % MB != 0 || p->n > 320)
+    return fail(c, JP_ERR_ARG, "N%MB: n must be a positive multiple of 8 (and at most 320)");
+  if (p->prec < 0 || p->prec > 3 || !(p->u0 > 0) || !(p->re > 0))
+    return fail(c, JP_ERR_ARG, "bad precision, u0 or Re");
+  if (p->l1_mode < 0 || p->l1_mode > 2 || p->l2_mode < 0 || p->l2_mode > 2)
+    return fail(c, JP_ERR_ARG, "refinement mode must be all, slab or sensor");
+  if (!(p->frac1 > 0 && p->frac1 <= 1) || !(p->frac2 > 0 && p->frac2 <= 1))
+    return fail(c, JP_ERR_ARG, "frac1 and frac2 must lie in (0, 1]");
+  c->device = p->device;
+  c->prec = p->prec;
+  c->u0 = p->u0;
+  c->re = p->re;
+  for (int i = 0; i < 3; ++i) c->phase[i] = p->phase[i];
+  HCK(c, hipSetDevice(c->device));
+  HCK(c, hipStreamCreate(&c->st));
+  return amr_create(p, c);
+}
+
+int jp_init_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
+  if (!c) return JP_ERR_ARG;
+  if (c->kind != 1) return fail(c, JP_ERR_STATE, "not an AMR context");
+  HCK(c, hipSetDevice(c->device));
+  return amr_init_with(c, sen0, sen1_by_pos);
+}
+
+int jp_regrid_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
+  if (!c) return JP_ERR_ARG;
+  if (c->kind != 1) return fail(c, JP_ERR_STATE, "not an AMR context");
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_regrid before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  return amr_regrid_with(c, sen0, sen1_by_pos);
+}
+
+int jp_regrid(jp_ctx* c) { return jp_regrid_with_sensor(c, nullptr, nullptr); }
+
+int jp_get_counts(jp_ctx* c, jp_counts* out) {
+  if (!c || !out) return JP_ERR_ARG;
+  if (c->kind != 1) return fail(c, JP_ERR_STATE, "not an AMR context");
+  return amr_counts(c, out);
+}
+
+int jp_get_table(jp_ctx* c, int which, void* dst, long cap, long* n) {
+  if (!c || (!dst && cap > 0)) return JP_ERR_ARG;
+  if (c->kind != 1) return fail(c, JP_ERR_STATE, "not an AMR context");
+  HCK(c, hipSetDevice(c->device));
+  return amr_table(c, which, dst, cap, n);
+}
+
 int jp_init(jp_ctx* c) {
   if (!c) return JP_ERR_ARG;
   HCK(c, hipSetDevice(c->device));
@@ -140,13 +193,53 @@ int jp_init(jp_ctx* c) {
   return JP_OK;
 }
 
-int jp_step(jp_ctx* c, long n) {
-  if (!c) return JP_ERR_ARG;
-  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_step before jp_init");
+static int eager_steps(jp_ctx* c, long n) {
   for (long s = 0; s < n; ++s) {
     int r = c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
     if (r) return r;
   }
+  if (n % 2) drop_graph(c);  // the buffers changed roles: a captured pair no longer applies
+  return JP_OK;
+}
+
+// Captures two steps (the ping-pong buffers return to their roles) into the context's graph.
+static int ensure_graph(jp_ctx* c) {
+  if (c->gexec) return JP_OK;
+  hipGraph_t g;
+  HCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  const int rc = eager_steps(c, 2);
+  hipError_t ec = hipStreamEndCapture(c->st, &g);
+  if (rc) return rc;
+  if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
+  ec = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ec != hipSuccess) {
+    c->gexec = nullptr;
+    return fail(c, JP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+  }
+  HCK(c, hipGraphUpload(c->gexec, c->st));
+  return JP_OK;
+}
+
+int jp_step(jp_ctx* c, long n) {
+  if (!c) return JP_ERR_ARG;
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_step before jp_init");
+  if (!c->use_graph) return eager_steps(c, n);
+  if (n >= 2) {
+    if (int r = ensure_graph(c)) return r;
+    for (long p = 0; p < n / 2; ++p) HCK(c, hipGraphLaunch(c->gexec, c->st));
+  }
+  if (n % 2) {
+    drop_graph(c);  // an odd step swaps the buffers the graph was captured with
+    return eager_steps(c, 1);
+  }
+  return JP_OK;
+}
+
+int jp_use_graph(jp_ctx* c, int on) {
+  if (!c) return JP_ERR_ARG;
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graph(c);
   return JP_OK;
 }
 
@@ -247,50 +340,49 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
   if (!c || !res || steps < 1 || warmup < 0) return JP_ERR_ARG;
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_bench before jp_init");
   HCK(c, hipSetDevice(c->device));
-  if (int r = jp_step(c, warmup)) return r;
+  // src/tgv3d_amr.cu:839-864: a dynamic AMR run regrids every `adapt` steps, in warm-up and timed loop
+  const bool dyn = c->kind == 1 && amr_dynamic(c) && c->amr.adapt > 0;
+  const long adapt = dyn ? c->amr.adapt : 0;
+  for (long w = 0; w < warmup; ++w) {
+    if (dyn && w % adapt == 0)
+      if (int r = amr_regrid_with(c, nullptr, nullptr)) return r;
+    if (int r = jp_step(c, 1)) return r;
+  }
   HCK(c, hipStreamSynchronize(c->st));
+  auto step_work = [&]() { return c->kind == 1 ? amr_work(c) : double(c->d.cells); };
   hipEvent_t e0, e1;
   HCK(c, hipEventCreate(&e0));
   HCK(c, hipEventCreate(&e1));
-  long done = 0;
-  int rc = JP_OK;
+  double work = 0;
   if (use_graph) {
-    // two steps per graph so the ping-pong buffers return to their starting roles
+    // two steps per graph so the ping-pong buffers return to their starting roles; no regrid inside
     if (steps % 2) steps += 1;
-    drop_graph(c);
-    hipGraph_t g;
-    HCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    rc = jp_step(c, 2);
-    hipError_t ec = hipStreamEndCapture(c->st, &g);
-    if (rc) return rc;
-    if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
-    HCK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    HCK(c, hipGraphUpload(c->gexec, c->st));
+    if (int r = ensure_graph(c)) return r;
     HCK(c, hipStreamSynchronize(c->st));
     HCK(c, hipEventRecord(e0, c->st));
     for (long r = 0; r < steps / 2; ++r) HCK(c, hipGraphLaunch(c->gexec, c->st));
     HCK(c, hipEventRecord(e1, c->st));
-    done = steps;
+    work = step_work() * double(steps);
   } else {
     HCK(c, hipEventRecord(e0, c->st));
-    rc = jp_step(c, steps);
-    if (rc) return rc;
+    for (long s = 0; s < steps; ++s) {
+      if (dyn && s % adapt == 0)
+        if (int r = amr_regrid_with(c, nullptr, nullptr)) return r;
+      if (int r = eager_steps(c, 1)) return r;
+      work += step_work();
+    }
     HCK(c, hipEventRecord(e1, c->st));
-    done = steps;
   }
   HCK(c, hipEventSynchronize(e1));
   float ms = 0;
   HCK(c, hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  double per_step_work = double(c->d.cells);
-  if (c->kind == 1) per_step_work = double(c->amr.cnb0 + 2L * c->amr.nf1 + 4L * c->amr.nf2) * MB3;
-  res->steps = done;
+  res->steps = steps;
   res->ms_total = ms;
-  res->ms_per_step = ms / double(done);
-  res->work = per_step_work * double(done);
-  res->glups = res->work / (double(ms) * 1e6);
+  res->ms_per_step = ms / double(steps);
+  res->work = work;
+  res->glups = work / (double(ms) * 1e6);
   res->bytes_per_update = 38.0 * double(prec_bytes(c->prec));
   res->gbps = res->glups * res->bytes_per_update;
   res->pool_gb = double(c->pool_bytes) / double(1ull << 30);
@@ -311,6 +403,7 @@ void jp_destroy(jp_ctx* c) {
   drop_graph(c);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto& a : c->allocs) (void)hipFree(a.first);
+  if (c->amr.hcnt) (void)hipHostFree(c->amr.hcnt);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

@@ -14,7 +14,7 @@ namespace jp {
 // Device state of the 3-level block AMR (tables follow src/tgv3d_amr.cu:692-709).
 struct AmrState {
   int N = 0, cgx = 0, cnb0 = 0, l1gx = 0, N1 = 0, N2 = 0, NG1 = 0;
-  int l1_mode = 0, l2_mode = 0, skipprep = 0, skiprestr = 0;
+  int l1_mode = 0, l2_mode = 0, skipprep = 0, skiprestr = 0, adapt = 100;
   double frac1 = 0.35, frac2 = 0.35, budget = 0;
   int cap_nref0 = 0, cap1 = 0, cap_nref1 = 0, cap2 = 0;
   long hw_nf1 = 0, hw_nf2 = 0;
@@ -26,7 +26,8 @@ struct AmrState {
   int *skin0 = nullptr, *skin1 = nullptr, *o_ref0 = nullptr, *o_fof0 = nullptr, *o_l1grid = nullptr;
   int *o_ref1 = nullptr, *o_fof1 = nullptr, *rank = nullptr, *deep = nullptr, *flags = nullptr;
   int *nb1 = nullptr, *nb2 = nullptr;  // 27 same-level neighbour tiles per fine tile (-1 = absent)
-  float *sen0 = nullptr, *sen1 = nullptr;
+  float *sen0 = nullptr, *sen1 = nullptr, *sen1pos = nullptr, *thr = nullptr;
+  uint32_t* slots = nullptr;  // ghost fill table
   int* dcnt = nullptr;  // device-resident counts and flags (see amr.hip)
   int* hcnt = nullptr;  // pinned host mirror of dcnt
 };
@@ -36,7 +37,7 @@ struct AmrState {
 struct jp_ctx {
   int kind = 0;  // 0 uniform, 1 AMR
   int device = 0, prec = 0, variant = 0;
-  bool initialised = false;
+  bool initialised = false, use_graph = false;
   hipStream_t st = nullptr;
   std::string err;
   jp_derived d{};
@@ -68,6 +69,13 @@ double default_scale(double u0);
 int amr_init(jp_ctx* c);
 int amr_base_step(jp_ctx* c, hipStream_t st);
 int amr_composite(jp_ctx* c, void* field);
+int amr_create(const jp_amr_params* p, jp_ctx* c);
+int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
+int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
+int amr_counts(jp_ctx* c, jp_counts* o);
+int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n);
+double amr_work(const jp_ctx* c);
+bool amr_dynamic(const jp_ctx* c);
 }  // namespace jp
 
 #define HCK(c, x)                                                                                 \

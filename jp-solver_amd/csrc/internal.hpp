@@ -24,6 +24,9 @@ inline Dense make_dense(int N, uint64_t pad) {
   d.divN = make_fastdiv(uint32_t(N));
   return d;
 }
+template <class R> struct alignas(sizeof(R) * 4) V4 {
+  R a, b, c, d;
+};
 inline size_t prec_bytes(int prec) { return prec == 3 ? 8 : (prec == 0 ? 4 : 2); }
 
 // ---- dense lattice (uniform.hip) ----

@@ -12,10 +12,6 @@
 
 namespace jp {
 
-template <class R> struct alignas(sizeof(R) * 4) V4 {
-  R a, b, c, d;
-};
-
 // Source coordinate arrays for c in {-1, 0, +1} (index c + 1): source = destination - c, periodic.
 struct Nbr {
   uint32_t xs[3], ys[3], zs[3];

@@ -32,6 +32,29 @@ class UniformParams(ctypes.Structure):
                 ("phase", ctypes.c_double * 3), ("scale", ctypes.c_double), ("device", ctypes.c_int)]
 
 
+class AmrParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("re", ctypes.c_double), ("u0", ctypes.c_double), ("prec", ctypes.c_int),
+                ("phase", ctypes.c_double * 3), ("l1_mode", ctypes.c_int), ("l2_mode", ctypes.c_int),
+                ("frac1", ctypes.c_double), ("frac2", ctypes.c_double), ("adapt", ctypes.c_int),
+                ("maxgb", ctypes.c_double), ("skipprep", ctypes.c_int), ("skiprestr", ctypes.c_int),
+                ("scale", ctypes.c_double * 3), ("device", ctypes.c_int)]
+
+
+COUNT_NAMES = ["cnb0", "nref0", "nf1", "nref1", "nf2", "n_skin0", "n_skin1", "cap_nref0", "cap1", "cap_nref1",
+               "cap2", "hw_nf1", "hw_nf2", "ndeep"]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_long) for k in COUNT_NAMES]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in COUNT_NAMES}
+
+
+TABLES = ["ref0", "fof0", "rl0", "l1pcb", "l1oct", "f1pos", "l1grid", "ref1", "fof1", "rl1", "l2pl1", "l2oct",
+          "skin0", "skin1", "sen0", "sen1"]
+
+
 class Derived(ctypes.Structure):
     _fields_ = [("nu", ctypes.c_double), ("tau0", ctypes.c_double), ("tau1", ctypes.c_double),
                 ("tau2", ctypes.c_double), ("omega0", ctypes.c_double), ("omega1", ctypes.c_double),
@@ -73,6 +96,11 @@ def lib():
             "jp_population_bytes": (i, [vp, i, ctypes.POINTER(sz)]),
             "jp_sync": (i, [vp]), "jp_last_error": (ctypes.c_char_p, [vp]), "jp_destroy": (None, [vp]),
             "jp_probe_copy": (i, [i, sz, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+            "jp_amr_create": (i, [ctypes.POINTER(AmrParams), ctypes.POINTER(vp)]),
+            "jp_regrid": (i, [vp]), "jp_init_with_sensor": (i, [vp, vp, vp]),
+            "jp_regrid_with_sensor": (i, [vp, vp, vp]),
+            "jp_get_counts": (i, [vp, ctypes.POINTER(Counts)]),
+            "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -179,3 +207,50 @@ class Uniform(_Ctx):
         rc = lib().jp_uniform_create(ctypes.byref(p), ctypes.byref(h))
         self.h = h.value
         self._check(rc)
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Amr(_Ctx):
+    """3-level block AMR TGV on the GPU (include/jp.h jp_amr_create)."""
+
+    def __init__(self, n=32, re=800.0, u0=0.05, l1=MODE_ALL, l2=MODE_ALL, frac1=0.35, frac2=0.35, adapt=100,
+                 maxgb=0.0, prec=PREC_FP32, skipprep=0, skiprestr=0, phase=(0.0, 0.0, 0.0), scales=(0, 0, 0),
+                 device=0):
+        self.prec, self.n = prec, n
+        p = AmrParams(n=n, re=re, u0=u0, prec=prec, l1_mode=l1, l2_mode=l2, frac1=frac1, frac2=frac2, adapt=adapt,
+                      maxgb=maxgb, skipprep=skipprep, skiprestr=skiprestr, device=device)
+        for k in range(3):
+            p.phase[k] = phase[k]
+            p.scale[k] = scales[k]
+        h = ctypes.c_void_p()
+        rc = lib().jp_amr_create(ctypes.byref(p), ctypes.byref(h))
+        self.h = h.value
+        self._check(rc)
+
+    def init(self, sen0=None, sen1_by_pos=None):
+        a0, a1 = _fptr(sen0), _fptr(sen1_by_pos)
+        self._check(lib().jp_init_with_sensor(self.h, a0[1] if a0 else None, a1[1] if a1 else None))
+
+    def regrid(self, sen0=None, sen1_by_pos=None):
+        a0, a1 = _fptr(sen0), _fptr(sen1_by_pos)
+        self._check(lib().jp_regrid_with_sensor(self.h, a0[1] if a0 else None, a1[1] if a1 else None))
+
+    def counts(self):
+        c = Counts()
+        self._check(lib().jp_get_counts(self.h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def table(self, name):
+        which = TABLES.index(name)
+        n = ctypes.c_long()
+        dt = np.float32 if name.startswith("sen") else np.int32
+        cap = 1 << 24
+        buf = np.empty(cap, dtype=dt)
+        self._check(lib().jp_get_table(self.h, which, buf.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n)))
+        return buf[:n.value].copy()
