@@ -42,6 +42,71 @@ def cpu_baseline(target_s=15.0):
                       f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
 
 
+def bench_uniform(n, prec, steps, warmup=20):
+    import jpabi
+
+    g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=jpabi.PREC_NAMES[prec])
+    g.init()
+    r = g.bench(steps, warmup=warmup)
+    g.close()
+    r["frac_hbm"] = r["gbps"] / PEAK_HBM_GBPS
+    return r
+
+
+def bench_amr(n0, re, l1, l2, frac1, frac2, adapt, steps, prec="fp32", graph=False):
+    """AMR coarse steps timed as the reference's --bench (src/tgv3d_amr.cu:839-864): 20 warm-up steps,
+    regrids every `adapt` steps inside the timed loop for a sensor-driven run; work weighted 1:2:4."""
+    import jpabi
+
+    a = jpabi.Amr(n0, re=re, u0=0.05, l1=jpabi.MODE_NAMES[l1], l2=jpabi.MODE_NAMES[l2], frac1=frac1, frac2=frac2,
+                  adapt=adapt, prec=jpabi.PREC_NAMES[prec])
+    a.init()
+    r = a.bench(steps, warmup=20, graph=graph)
+    r["counts"] = a.counts()
+    a.close()
+    r["frac_hbm"] = r["gbps"] / PEAK_HBM_GBPS
+    return r
+
+
+def bench_ensemble(steps, graph=True, tune=True):
+    import jpabi
+
+    with open(os.path.join(ROOT, "jp-solver_amd", "ensemble32.json")) as f:
+        members = json.load(f)["members"]
+    e = jpabi.Ensemble(members)
+    e.init()
+    groups = e.autotune() if tune else None
+    r = e.bench(steps, warmup=4, graph=graph)
+    e.close()
+    r["groups"] = groups
+    r["frac_hbm"] = r["gbps"] / PEAK_HBM_GBPS
+    return r
+
+
+def extras(quick=False):
+    """Secondary workloads of the same metric (reported beside the headline, never as `value`)."""
+    out = {}
+    jobs = {
+        "uniform512_fp16": lambda: bench_uniform(512, "fp16", 300 if quick else 1000),
+        "uniform512_bf16": lambda: bench_uniform(512, "bf16", 300 if quick else 1000),
+        "amr_dynamic_96_384": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, 100 if quick else 300),
+        "amr_dynamic_96_384_fp16": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10,
+                                                     100 if quick else 300, prec="fp16"),
+        "amr_gold_64_256": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, 50 if quick else 200,
+                                             graph=True),
+        "ensemble32": lambda: bench_ensemble(20 if quick else 60),
+    }
+    for k, fn in jobs.items():
+        t0 = time.time()
+        try:
+            out[k] = fn()
+        except Exception as ex:  # an extra must not hide the headline measurement
+            out[k] = {"error": str(ex)}
+        out[k]["wall_s"] = time.time() - t0
+        print(f"# extra {k}: {out[k].get('glups', float('nan')):.3f} GLUPS", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +116,9 @@ def main():
     ap.add_argument("--prec", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--quick-extras", action="store_true", help="shorter secondary workloads")
+    ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -59,6 +127,13 @@ def main():
 
     import jpabi
 
+    if args.only:
+        fn = {"amr-dynamic": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps),
+              "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
+              "ensemble": lambda: bench_ensemble(args.steps),
+              "uniform": lambda: bench_uniform(args.n, args.prec, args.steps)}[args.only]
+        print(json.dumps({"workload": args.only, **fn()}), flush=True)
+        return
     prec = jpabi.PREC_NAMES[args.prec]
     n = args.n
     copy_gbps, _ = jpabi.probe_copy(4 << 30, iters=10)
@@ -97,6 +172,8 @@ def main():
         "extra": {"copy_kernel_gbps": copy_gbps, "bytes_per_update": bpu, "pool_gb": r["pool_gb"],
                   "diag_after": [E, ens, mass]},
     }
+    if not args.no_extras:
+        line["extra"]["workloads"] = extras(quick=args.quick_extras)
     if not args.no_cpu:
         t0 = time.time()
         line["cpu_baseline"] = cpu_baseline()

@@ -142,6 +142,30 @@ int jp_regrid(jp_ctx* c);
 int jp_init_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
 int jp_regrid_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
 
+/* One member of a batched ensemble (not in the reference). */
+typedef struct {
+  int n;            /* lattice edge (uniform) or coarse edge N0 (AMR); multiple of 8 */
+  double re, u0;
+  double phase[3];  /* TGV phase shift per axis */
+  int amr;          /* 0 uniform lattice, 1 three-level AMR */
+  int l1_mode, l2_mode;
+  double frac1, frac2;
+  int adapt;        /* regrid period of a sensor-driven AMR member in jp_bench */
+} jp_member;
+
+/* Batched ensemble (not in the reference; built for many small lattices whose single launches would
+ * leave the GPU idle): `count` independent TGV lattices, uniform and AMR mixed, sharing one stream. One
+ * jp_step advances every member by one step (uniform) or one coarse step (AMR, 1:2:4 subcycle) with one
+ * persistent launch per substep kind over all members' tiles; jp_use_graph captures it in a hipGraph;
+ * jp_regrid regrids every sensor-driven AMR member; jp_bench reports the aggregate subcycle-weighted
+ * throughput. Members are reachable with jp_ensemble_member for diagnostics and the test hooks. */
+int jp_ensemble_create(const jp_member* members, int count, int prec, double maxgb, int device, jp_ctx** out);
+/* Borrowed handle of member `index` (owned by the ensemble, valid until jp_destroy of the ensemble). */
+int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
+/* Picks the workgroup count of every batched launch by timing candidates on the current state (each
+ * timed kernel is re-run on unchanged inputs, so the state is unchanged); result in groups[9]. */
+int jp_autotune(jp_ctx* ens, int* groups);
+
 int jp_get_counts(jp_ctx* c, jp_counts* out);
 /* Copies table `which` (JP_TABLE_*) to dst (capacity cap elements); *n receives its length. */
 int jp_get_table(jp_ctx* c, int which, void* dst, long cap, long* n);

@@ -1,0 +1,233 @@
+// Device bodies of the AMR kernels, shared by the per-context launches (amr.hip) and the batched
+// ensemble launches (ensemble.hip). Each body computes one cell or one tile of one level.
+#pragma once
+#include "amr_internal.hpp"
+
+namespace jp {
+
+__device__ inline int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+// Trilinear base and weight of the fine coordinate g on the parent grid: x = (g + 1/2)/2 - 1/2,
+// base = floor(x), t = x - base (src/tgv3d_amr.cu:71-73). Exact in integers: g = 2m gives (m - 1, 3/4),
+// g = 2m + 1 gives (m, 1/4).
+__device__ inline void tri(int g, int& base, float& t) {
+  const int m = g >> 1;
+  if (g & 1) {
+    base = m;
+    t = 0.25f;
+  } else {
+    base = m - 1;
+    t = 0.75f;
+  }
+}
+__device__ inline uint32_t dense_cell(const AmrGeo& G, int ci, int cj, int ck) {
+  return (uint32_t(ck) * G.N0 + cj) * G.N0 + ci;
+}
+__device__ inline uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
+  const int bx = cb % G.cgx, by = (cb / G.cgx) % G.cgx, bz = cb / (G.cgx * G.cgx);
+  return dense_cell(G, bx * MB + (cell & 7), by * MB + ((cell >> 3) & 7), bz * MB + (cell >> 6));
+}
+__device__ inline size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
+
+template <class S> __device__ inline void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
+                                                    typename S::real* out) {
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    out[q] = S::template load<q>(f[tidx(b, q, cell)], dec);
+  });
+}
+template <class S> __device__ inline void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
+                                                     float dec, typename S::real* out) {
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    out[q] = S::template load<q>(f[uint64_t(q) * plane + idx], dec);
+  });
+}
+
+// BGK collision of pulled populations, stored to tile b (src/tgv3d_amr.cu:268-273).
+template <class S> __device__ inline void collide_store(typename S::real* fin, typename S::real om,
+                                                        typename S::raw* __restrict__ fb, int b, int cell, float enc) {
+  using R = typename S::real;
+  const Mom<R> m = moments(fin);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    fb[tidx(b, q, cell)] = S::template store<q>(fin[q] - om * (fin[q] - feq<q>(e)), enc);
+  });
+}
+
+// Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
+// 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
+// to this level); missing sources come from the ghost record.
+// snb: the tile's neighbour entries, staged in LDS by the caller.
+template <class S>
+__device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                      const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
+                                      float enc, float dec, int b, int cell) {
+  using R = typename S::real;
+  const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
+  const R* __restrict__ g = gh + size_t(b) * kGhostSlots;
+  R fin[Q];
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+    if constexpr (q == 0) {
+      fin[0] = S::template load<0>(fa[tidx(b, 0, cell)], dec);
+    } else {
+      const int si = i - cx, sj = j - cy, sk = k - cz;
+      const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
+      const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
+      const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
+      const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+      if (t >= 0) fin[q] = S::template load<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
+      else fin[q] = g[ghost_slot(q, i, j, k)];
+    }
+  });
+  collide_store<S>(fin, om, fb, b, cell, enc);
+}
+
+// prep = feq + nqs * regularize(f - feq) (src/tgv3d_amr.cu:160-171), listed tiles only.
+template <class S, bool kDense>
+__device__ inline void prep_cell(const typename S::raw* __restrict__ f, typename S::real* __restrict__ pr,
+                                 const AmrGeo& G, float nqs, float dec, int b, int cell) {
+  using R = typename S::real;
+  R v[Q], nq[Q];
+  uint32_t di = 0;
+  if constexpr (kDense) {
+    di = dense_of_tile(G, b, cell);
+    load_dense<S>(f, G.g0.plane, di, dec, v);
+  } else {
+    load_tile<S>(f, b, cell, dec, v);
+  }
+  const Mom<R> m = moments(v);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    nq[q] = v[q] - feq<q>(e);
+  });
+  regularize(nq);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const R val = feq<q>(e) + R(nqs) * nq[q];
+    if constexpr (kDense) pr[uint64_t(q) * G.g0.plane + di] = val;
+    else pr[tidx(b, q, cell)] = val;
+  });
+}
+
+// Ghost record of level-1 tiles from prep0 (src/tgv3d_amr.cu:122-132, 173-186): for each slot whose
+// source position lies in an unrefined coarse tile, the trilinear interpolation of that direction.
+template <class R>
+__device__ inline void fill_l1_tile(const R* __restrict__ prep0, R* __restrict__ gh1, const int* __restrict__ l1pcb,
+                                    const int* __restrict__ l1oct, const int* __restrict__ ref0,
+                                    const uint32_t* __restrict__ slots, const AmrGeo& G, int fb, int tid, int nthr) {
+  const int cb = l1pcb[fb], oct = l1oct[fb];
+  const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
+  const int ox = cbx * 2 * MB + (oct & 1) * MB, oy = cby * 2 * MB + ((oct >> 1) & 1) * MB, oz = cbz * 2 * MB + (oct >> 2) * MB;
+  for (int s = tid; s < kGhostSlots; s += nthr) {
+    const uint32_t e = slots[s];
+    const int q = (e >> 9) & 31;
+    const int gfi = pmod(ox + int((e >> 14) & 15) - 1, G.N1), gfj = pmod(oy + int((e >> 18) & 15) - 1, G.N1),
+              gfk = pmod(oz + int((e >> 22) & 15) - 1, G.N1);
+    const int scb = ((gfk / (2 * MB)) * G.cgx + (gfj / (2 * MB))) * G.cgx + (gfi / (2 * MB));
+    if (ref0[scb]) continue;
+    int ib, jb, kb;
+    float tx, ty, tz;
+    tri(gfi, ib, tx);
+    tri(gfj, jb, ty);
+    tri(gfk, kb, tz);
+    const R* __restrict__ p = prep0 + uint64_t(q) * G.g0.plane;
+    R out = R(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+      const uint32_t idx = dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0));
+      out += R(wgt) * p[idx];
+    }
+    gh1[size_t(fb) * kGhostSlots + s] = out;
+  }
+}
+
+// Ghost record of level-2 tiles from prep1 over the scattered level 1, dropped corners renormalised
+// (src/tgv3d_amr.cu:133-144, 187-202).
+template <class R>
+__device__ inline void fill_l2_tile(const R* __restrict__ prep1, R* __restrict__ gh2, const int* __restrict__ l2pl1,
+                                    const int* __restrict__ l2oct, const int* __restrict__ f1pos,
+                                    const int* __restrict__ l1grid, const int* __restrict__ ref1,
+                                    const uint32_t* __restrict__ slots, const AmrGeo& G, int b, int tid, int nthr) {
+  const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
+  const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
+  const int ox = px * 2 * MB + (oct & 1) * MB, oy = py * 2 * MB + ((oct >> 1) & 1) * MB, oz = pz * 2 * MB + (oct >> 2) * MB;
+  for (int s = tid; s < kGhostSlots; s += nthr) {
+    const uint32_t e = slots[s];
+    const int q = (e >> 9) & 31;
+    const int g2i = pmod(ox + int((e >> 14) & 15) - 1, G.N2), g2j = pmod(oy + int((e >> 18) & 15) - 1, G.N2),
+              g2k = pmod(oz + int((e >> 22) & 15) - 1, G.N2);
+    const int lbs = l1grid[((g2k / (2 * MB)) * G.l1gx + (g2j / (2 * MB))) * G.l1gx + (g2i / (2 * MB))];
+    if (lbs >= 0 && ref1[lbs]) continue;
+    int ib, jb, kb;
+    float tx, ty, tz, wsum = 0.f;
+    tri(g2i, ib, tx);
+    tri(g2j, jb, ty);
+    tri(g2k, kb, tz);
+    R out = R(0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
+      const int ci = pmod(ib + di, G.N1), cj = pmod(jb + dj, G.N1), ck = pmod(kb + dk, G.N1);
+      const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
+      if (lb < 0) continue;
+      const int cc = ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7);
+      wsum += wgt;
+      out += R(wgt) * prep1[tidx(lb, q, cc)];
+    }
+    if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
+    gh2[size_t(b) * kGhostSlots + s] = out;
+  }
+}
+
+// Restriction (src/tgv3d_amr.cu:275-298, 358-381): each covered parent cell = mean of its 8 children,
+// then feq + nqi * regularize(mean - feq). One lane per parent cell.
+template <class S, bool kToDense>
+__device__ inline void restrict_cell(typename S::raw* __restrict__ cf, const typename S::raw* __restrict__ ff,
+                                     const int* __restrict__ rl, const int* __restrict__ fof, const AmrGeo& G, float nqi,
+                                     float cenc, float fdec, int r, int cell) {
+  using R = typename S::real;
+  const int cb = rl[r];
+  const int lci = cell & 7, lcj = (cell >> 3) & 7, lck = cell >> 6;
+  const int oct = (lck / HALF) * 4 + (lcj / HALF) * 2 + (lci / HALF), fb = fof[cb] + oct;
+  const int fi0 = 2 * (lci % HALF), fj0 = 2 * (lcj % HALF), fk0 = 2 * (lck % HALF);
+  R avg[Q], nq[Q];
+  sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0); });
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
+    const int sc = ((fk0 + dk) * MB + (fj0 + dj)) * MB + (fi0 + di);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      avg[q] += R(0.125) * S::template load<q>(ff[tidx(fb, q, sc)], fdec);
+    });
+  }
+  const Mom<R> m = moments(avg);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    nq[q] = avg[q] - feq<q>(e);
+  });
+  regularize(nq);
+  const uint32_t di = kToDense ? dense_of_tile(G, cb, cell) : 0;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
+    if constexpr (kToDense) cf[uint64_t(q) * G.g0.plane + di] = v;
+    else cf[tidx(cb, q, cell)] = v;
+  });
+}
+
+}  // namespace jp

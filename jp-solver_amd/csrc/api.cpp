@@ -162,6 +162,11 @@ int jp_init_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos) 
 
 int jp_regrid_with_sensor(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   if (!c) return JP_ERR_ARG;
+  if (c->kind == 2 && !sen0 && !sen1_by_pos) {
+    if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_regrid before jp_init");
+    HCK(c, hipSetDevice(c->device));
+    return ens_regrid(c);
+  }
   if (c->kind != 1) return fail(c, JP_ERR_STATE, "not an AMR context");
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_regrid before jp_init");
   HCK(c, hipSetDevice(c->device));
@@ -183,9 +188,38 @@ int jp_get_table(jp_ctx* c, int which, void* dst, long cap, long* n) {
   return amr_table(c, which, dst, cap, n);
 }
 
+int jp_ensemble_create(const jp_member* members, int count, int prec, double maxgb, int device, jp_ctx** out) {
+  if (!members || !out || count < 1) return JP_ERR_ARG;
+  auto* c = new jp_ctx();
+  *out = c;
+  c->kind = 2;
+  if (prec < 0 || prec > 3) return fail(c, JP_ERR_ARG, "bad precision");
+  c->prec = prec;
+  c->device = device;
+  HCK(c, hipSetDevice(device));
+  HCK(c, hipStreamCreate(&c->st));
+  return ens_create(members, count, prec, maxgb, device, c);
+}
+
+int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member) {
+  if (!ens || !member) return JP_ERR_ARG;
+  if (ens->kind != 2) return fail(ens, JP_ERR_STATE, "not an ensemble");
+  *member = ens_member(ens, index);
+  return *member ? JP_OK : fail(ens, JP_ERR_ARG, "member index out of range");
+}
+
+int jp_autotune(jp_ctx* ens, int* groups) {
+  if (!ens) return JP_ERR_ARG;
+  if (ens->kind != 2) return fail(ens, JP_ERR_STATE, "autotuning applies to the batched ensemble");
+  if (!ens->initialised) return fail(ens, JP_ERR_STATE, "jp_autotune before jp_init");
+  HCK(ens, hipSetDevice(ens->device));
+  return ens_autotune(ens, groups);
+}
+
 int jp_init(jp_ctx* c) {
   if (!c) return JP_ERR_ARG;
   HCK(c, hipSetDevice(c->device));
+  if (c->kind == 2) return ens_init(c);
   if (c->kind == 1) return amr_init(c);
   HCK(c, dense_init(c->prec, c->c0a, c->g0, c->u0, c->d.kf, c->phase, float(c->d.scale[0]), c->st));
   HCK(c, hipStreamSynchronize(c->st));
@@ -195,7 +229,7 @@ int jp_init(jp_ctx* c) {
 
 static int eager_steps(jp_ctx* c, long n) {
   for (long s = 0; s < n; ++s) {
-    int r = c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
+    int r = c->kind == 2 ? ens_step_once(c) : c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
     if (r) return r;
   }
   if (n % 2) drop_graph(c);  // the buffers changed roles: a captured pair no longer applies
@@ -224,6 +258,8 @@ static int ensure_graph(jp_ctx* c) {
 int jp_step(jp_ctx* c, long n) {
   if (!c) return JP_ERR_ARG;
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_step before jp_init");
+  if (c->kind == 2)
+    if (int r = ens_check(c)) return r;
   if (!c->use_graph) return eager_steps(c, n);
   if (n >= 2) {
     if (int r = ensure_graph(c)) return r;
@@ -245,6 +281,7 @@ int jp_use_graph(jp_ctx* c, int on) {
 
 int jp_diag(jp_ctx* c, double out[3]) {
   if (!c || !out) return JP_ERR_ARG;
+  if (c->kind == 2) return fail(c, JP_ERR_STATE, "diagnostics are per member (jp_ensemble_member)");
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_diag before jp_init");
   HCK(c, hipSetDevice(c->device));
   const int nf = c->kind == 1 ? c->amr.N2 : c->d.n;
@@ -269,6 +306,7 @@ int jp_get_derived(jp_ctx* c, jp_derived* d) {
 
 int jp_population_bytes(jp_ctx* c, int level, size_t* bytes) {
   if (!c || !bytes) return JP_ERR_ARG;
+  if (c->kind == 2) return fail(c, JP_ERR_STATE, "populations are per member (jp_ensemble_member)");
   long tiles;
   if (level == 0) tiles = c->d.cells / MB3;
   else if (c->kind == 1 && level == 1) tiles = c->amr.nf1;
@@ -341,15 +379,19 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_bench before jp_init");
   HCK(c, hipSetDevice(c->device));
   // src/tgv3d_amr.cu:839-864: a dynamic AMR run regrids every `adapt` steps, in warm-up and timed loop
-  const bool dyn = c->kind == 1 && amr_dynamic(c) && c->amr.adapt > 0;
-  const long adapt = dyn ? c->amr.adapt : 0;
+  const long adapt = c->kind == 1 ? ((amr_dynamic(c) && c->amr.adapt > 0) ? c->amr.adapt : 0)
+                                   : (c->kind == 2 ? ens_adapt(c) : 0);
+  const bool dyn = adapt > 0;
+  auto regrid = [&]() { return c->kind == 2 ? ens_regrid(c) : amr_regrid_with(c, nullptr, nullptr); };
   for (long w = 0; w < warmup; ++w) {
     if (dyn && w % adapt == 0)
-      if (int r = amr_regrid_with(c, nullptr, nullptr)) return r;
+      if (int r = regrid()) return r;
     if (int r = jp_step(c, 1)) return r;
   }
+  if (c->kind == 2)
+    if (int r = ens_check(c)) return r;
   HCK(c, hipStreamSynchronize(c->st));
-  auto step_work = [&]() { return c->kind == 1 ? amr_work(c) : double(c->d.cells); };
+  auto step_work = [&]() { return c->kind == 2 ? ens_work(c) : c->kind == 1 ? amr_work(c) : double(c->d.cells); };
   hipEvent_t e0, e1;
   HCK(c, hipEventCreate(&e0));
   HCK(c, hipEventCreate(&e1));
@@ -367,7 +409,7 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
     HCK(c, hipEventRecord(e0, c->st));
     for (long s = 0; s < steps; ++s) {
       if (dyn && s % adapt == 0)
-        if (int r = amr_regrid_with(c, nullptr, nullptr)) return r;
+        if (int r = regrid()) return r;
       if (int r = eager_steps(c, 1)) return r;
       work += step_work();
     }
@@ -402,6 +444,7 @@ void jp_destroy(jp_ctx* c) {
   (void)hipSetDevice(c->device);
   drop_graph(c);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  ens_destroy(c);
   for (auto& a : c->allocs) (void)hipFree(a.first);
   if (c->amr.hcnt) (void)hipHostFree(c->amr.hcnt);
   if (c->st) (void)hipStreamDestroy(c->st);

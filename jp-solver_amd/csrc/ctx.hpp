@@ -11,6 +11,8 @@
 
 namespace jp {
 
+struct EnsState;
+
 // Device state of the 3-level block AMR (tables follow src/tgv3d_amr.cu:692-709).
 struct AmrState {
   int N = 0, cgx = 0, cnb0 = 0, l1gx = 0, N1 = 0, N2 = 0, NG1 = 0;
@@ -35,7 +37,7 @@ struct AmrState {
 }  // namespace jp
 
 struct jp_ctx {
-  int kind = 0;  // 0 uniform, 1 AMR
+  int kind = 0;  // 0 uniform, 1 AMR, 2 batched ensemble
   int device = 0, prec = 0, variant = 0;
   bool initialised = false, use_graph = false;
   hipStream_t st = nullptr;
@@ -51,6 +53,7 @@ struct jp_ctx {
   std::vector<std::pair<void*, size_t>> allocs;
   size_t pool_bytes = 0;
   jp::AmrState amr;
+  jp::EnsState* ens = nullptr;  // batched ensemble (kind 2)
 };
 
 namespace jp {
@@ -76,6 +79,18 @@ int amr_counts(jp_ctx* c, jp_counts* o);
 int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n);
 double amr_work(const jp_ctx* c);
 bool amr_dynamic(const jp_ctx* c);
+// batched ensemble (ensemble_host.cpp)
+int ens_create(const jp_member* mem, int count, int prec, double maxgb, int device, jp_ctx* c);
+int ens_init(jp_ctx* c);
+int ens_check(jp_ctx* c);
+int ens_step_once(jp_ctx* c);
+int ens_regrid(jp_ctx* c);
+bool ens_dynamic(const jp_ctx* c);
+int ens_adapt(const jp_ctx* c);
+double ens_work(const jp_ctx* c);
+int ens_autotune(jp_ctx* c, int* out);
+void ens_destroy(jp_ctx* c);
+jp_ctx* ens_member(jp_ctx* c, int i);
 }  // namespace jp
 
 #define HCK(c, x)                                                                                 \

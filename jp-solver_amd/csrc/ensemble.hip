@@ -1,0 +1,132 @@
+// Batched ensemble kernels: one persistent launch per substep advances the same substep of every
+// member lattice (uniform lattices, AMR level 0, the fine levels of every AMR member). Work is a list of
+// (member, unit) items; each workgroup walks the list with a grid stride, so the launch shape (groups
+// per CU) is independent of the ensemble and is picked by the autotuner. The per-item work is the same
+// device body the single-context kernels run, so a member advanced in the batch is bit-identical to the
+// same member advanced alone.
+#include "amr_bodies.hpp"
+#include "dense_bodies.hpp"
+#include "ensemble.hpp"
+
+namespace jp {
+
+template <class S> __device__ inline const typename S::raw* cr(const void* p) {
+  return static_cast<const typename S::raw*>(p);
+}
+template <class S> __device__ inline typename S::raw* wr(void* p) { return static_cast<typename S::raw*>(p); }
+
+// items: (member, 256-cell chunk)
+template <class S>
+__global__ __launch_bounds__(256) void kb_dense_step(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+                                                     int n, int p0) {
+  using R = typename S::real;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    dense_step_cell<S>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
+                       uint32_t(w.y) * 256 + threadIdx.x);
+  }
+}
+// items: (member, index into the skin list); level 0 prep reads the dense lattice
+template <class S, int kLevel>
+__global__ __launch_bounds__(512) void kb_prep(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
+                                               int p) {
+  using R = typename S::real;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    if constexpr (kLevel == 0)
+      prep_cell<S, true>(cr<S>(d.c0[p]), static_cast<R*>(d.prep0), d.G, d.nqs01, d.dec[0], d.skin0[w.y], threadIdx.x);
+    else
+      prep_cell<S, false>(cr<S>(d.f1[p]), static_cast<R*>(d.prep1), d.G, d.nqs12, d.dec[1], d.skin1[w.y], threadIdx.x);
+  }
+}
+// items: (member, fine tile)
+template <class R, int kLevel>
+__global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n) {
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    if constexpr (kLevel == 1)
+      fill_l1_tile<R>(static_cast<const R*>(d.prep0), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.ref0, d.slots, d.G, w.y,
+                      threadIdx.x, 256);
+    else
+      fill_l2_tile<R>(static_cast<const R*>(d.prep1), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.ref1,
+                      d.slots, d.G, w.y, threadIdx.x, 256);
+  }
+}
+// items: (member, fine tile)
+template <class S, int kLevel>
+__global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+                                                    int n, int p) {
+  using R = typename S::real;
+  __shared__ int snb[kNbr];
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
+    __syncthreads();  // the previous item's lanes are done with snb
+    if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(w.y) * kNbr + threadIdx.x];
+    __syncthreads();
+    if constexpr (kLevel == 1)
+      fine_step_cell<S>(cr<S>(d.f1[p]), wr<S>(d.f1[p ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]), d.enc[1],
+                        d.dec[1], w.y, threadIdx.x);
+    else
+      fine_step_cell<S>(cr<S>(d.f2[p]), wr<S>(d.f2[p ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]), d.enc[2],
+                        d.dec[2], w.y, threadIdx.x);
+  }
+}
+// items: (member, 2 * refined tile + half): 256 parent cells per item
+template <class S, int kLevel>
+__global__ __launch_bounds__(256) void kb_restrict(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
+                                                   int pc, int pf) {
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    const int r = w.y >> 1, cell = (w.y & 1) * 256 + threadIdx.x;
+    if constexpr (kLevel == 1)  // level 1 -> dense level 0
+      restrict_cell<S, true>(wr<S>(d.c0[pc]), cr<S>(d.f1[pf]), d.rl0, d.fof0, d.G, d.nqi10, d.enc[0], d.dec[1], r, cell);
+    else
+      restrict_cell<S, false>(wr<S>(d.f1[pc]), cr<S>(d.f2[pf]), d.rl1, d.fof1, d.G, d.nqi21, d.enc[1], d.dec[2], r, cell);
+  }
+}
+
+#define JP_DISPATCH(prec, CALL)                   \
+  switch (prec) {                                 \
+    case 0: { using S = FormF32; CALL; } break;   \
+    case 1: { using S = FormF16; CALL; } break;   \
+    case 2: { using S = FormBF16; CALL; } break;  \
+    case 3: { using S = FormF64; CALL; } break;   \
+    default: return hipErrorInvalidValue;         \
+  }
+
+static inline dim3 grid_of(int n, int groups) { return dim3(unsigned(std::max(1, std::min(n, groups)))); }
+
+hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* items, int n, int groups, int pa, int pb,
+                      hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g = grid_of(n, groups);
+  switch (k) {
+    case EK_DENSE: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_dense_step<S>), g, dim3(256), 0, st, md, items, n, pa)); break;
+    case EK_PREP0: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_prep<S, 0>), g, dim3(512), 0, st, md, items, n, pa)); break;
+    case EK_PREP1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_prep<S, 1>), g, dim3(512), 0, st, md, items, n, pa)); break;
+    case EK_FILL1:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<typename S::real, 1>), g, dim3(256), 0, st, md, items, n));
+      break;
+    case EK_FILL2:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<typename S::real, 2>), g, dim3(256), 0, st, md, items, n));
+      break;
+    case EK_STEP1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 1>), g, dim3(512), 0, st, md, items, n, pa)); break;
+    case EK_STEP2: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 2>), g, dim3(512), 0, st, md, items, n, pa)); break;
+    case EK_RESTR1:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_restrict<S, 1>), g, dim3(256), 0, st, md, items, n, pa, pb));
+      break;
+    case EK_RESTR2:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_restrict<S, 2>), g, dim3(256), 0, st, md, items, n, pa, pb));
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace jp

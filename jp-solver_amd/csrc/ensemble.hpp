@@ -1,0 +1,31 @@
+// Batched ensemble: per-member device descriptor and the batched launch interface (ensemble.hip).
+#pragma once
+#include "amr_internal.hpp"
+
+namespace jp {
+
+// Everything a batched kernel needs about one member. Buffers that ping-pong are stored as pairs; the
+// launch passes the parity that selects the current one (all members of an ensemble swap in step).
+struct MemberDesc {
+  void* c0[2];
+  void* f1[2];
+  void* f2[2];
+  void *prep0, *prep1, *gh1, *gh2;
+  const int *ref0, *fof0, *rl0, *l1pcb, *l1oct, *f1pos, *l1grid, *ref1, *fof1, *rl1, *l2pl1, *l2oct;
+  const int *skin0, *skin1, *nb1, *nb2;
+  const uint32_t* slots;
+  Dense g0;
+  AmrGeo G;
+  double om[3];
+  float enc[3], dec[3];
+  float nqs01, nqi10, nqs12, nqi21;
+};
+
+enum EnsKernel { EK_DENSE = 0, EK_PREP0, EK_FILL1, EK_PREP1, EK_FILL2, EK_STEP2, EK_RESTR2, EK_STEP1, EK_RESTR1, EK_COUNT };
+
+// One batched launch of kernel k over n items with at most `groups` workgroups. pa: parity of the
+// buffer read (for restriction: the parent level's), pb: the fine level's parity (restriction only).
+hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* items, int n, int groups, int pa, int pb,
+                      hipStream_t st);
+
+}  // namespace jp

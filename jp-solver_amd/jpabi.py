@@ -101,6 +101,8 @@ def lib():
             "jp_regrid_with_sensor": (i, [vp, vp, vp]),
             "jp_get_counts": (i, [vp, ctypes.POINTER(Counts)]),
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
+            "jp_use_graph": (i, [vp, i]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
+            "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -191,6 +193,9 @@ class _Ctx:
         self._check(lib().jp_set_populations(self.h, level, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
                     if a.nbytes == nb else JP_ERR_ARG)
 
+    def use_graph(self, on=True):
+        self._check(lib().jp_use_graph(self.h, 1 if on else 0))
+
     def sync(self):
         self._check(lib().jp_sync(self.h))
 
@@ -254,3 +259,70 @@ class Amr(_Ctx):
         buf = np.empty(cap, dtype=dt)
         self._check(lib().jp_get_table(self.h, which, buf.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(n)))
         return buf[:n.value].copy()
+
+
+class Member(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("re", ctypes.c_double), ("u0", ctypes.c_double), ("phase", ctypes.c_double * 3),
+                ("amr", ctypes.c_int), ("l1_mode", ctypes.c_int), ("l2_mode", ctypes.c_int), ("frac1", ctypes.c_double),
+                ("frac2", ctypes.c_double), ("adapt", ctypes.c_int)]
+
+
+class _Borrowed(_Ctx):
+    """A member of an ensemble: owned by the ensemble, never destroyed through this handle."""
+
+    def __init__(self, h, prec, owner):
+        self.h, self.prec, self._owner = h, prec, owner
+
+    def close(self):
+        self.h = None
+
+    def counts(self):
+        return Amr.counts(self)
+
+    def table(self, name):
+        return Amr.table(self, name)
+
+
+class Ensemble(_Ctx):
+    """Batched ensemble of uniform and AMR TGV lattices (include/jp.h jp_ensemble_create)."""
+
+    def __init__(self, members, prec=PREC_FP32, maxgb=0.0, device=0):
+        self.prec = prec
+        arr = (Member * len(members))()
+        for i, m in enumerate(members):
+            arr[i].n = m["n"]
+            arr[i].re = m["re"]
+            arr[i].u0 = m["u0"]
+            for k in range(3):
+                arr[i].phase[k] = m.get("phase", (0.0, 0.0, 0.0))[k]
+            arr[i].amr = 1 if m.get("amr") else 0
+            arr[i].l1_mode = m.get("l1", MODE_SENSOR)
+            arr[i].l2_mode = m.get("l2", MODE_SENSOR)
+            arr[i].frac1 = m.get("frac1", 0.3)
+            arr[i].frac2 = m.get("frac2", 0.1)
+            arr[i].adapt = m.get("adapt", 10)
+        h = ctypes.c_void_p()
+        rc = lib().jp_ensemble_create(arr, len(members), prec, maxgb, device, ctypes.byref(h))
+        self.h = h.value
+        self._check(rc)
+        self.members = []
+        for i in range(len(members)):
+            mh = ctypes.c_void_p()
+            self._check(lib().jp_ensemble_member(self.h, i, ctypes.byref(mh)))
+            self.members.append(_Borrowed(mh.value, prec, self))
+
+    def close(self):
+        for m in getattr(self, "members", []):
+            m.close()
+        super().close()
+
+    def regrid(self):
+        self._check(lib().jp_regrid(self.h))
+
+    def use_graph(self, on=True):
+        self._check(lib().jp_use_graph(self.h, 1 if on else 0))
+
+    def autotune(self):
+        g = (ctypes.c_int * 9)()
+        self._check(lib().jp_autotune(self.h, g))
+        return list(g)

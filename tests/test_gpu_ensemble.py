@@ -1,0 +1,96 @@
+"""GPU checks of the batched ensemble: every member advanced by the batched launches is bit-identical to
+the same member advanced alone (single-context launches, which are themselves bit-exact with the
+oracle), with and without hipGraph replay, across regrids; autotuning leaves the state unchanged."""
+import numpy as np
+import pytest
+
+import jpabi
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [
+    {"n": 16, "re": 1600.0, "u0": 0.05, "amr": 0},
+    {"n": 24, "re": 900.0, "u0": 0.04, "amr": 0, "phase": (0.3, 1.1, 2.0)},
+    {"n": 16, "re": 800.0, "u0": 0.05, "amr": 1, "l1": jpabi.MODE_SLAB, "l2": jpabi.MODE_ALL},
+    {"n": 24, "re": 1600.0, "u0": 0.06, "amr": 1, "l1": jpabi.MODE_SENSOR, "l2": jpabi.MODE_SENSOR, "frac1": 0.4,
+     "frac2": 0.3, "adapt": 3, "phase": (1.0, 0.5, 0.25)},
+]
+
+
+def standalone(m, prec):
+    if m["amr"]:
+        return jpabi.Amr(m["n"], re=m["re"], u0=m["u0"], l1=m["l1"], l2=m["l2"], frac1=m.get("frac1", 0.3),
+                         frac2=m.get("frac2", 0.1), adapt=m.get("adapt", 10), prec=prec, maxgb=16.0,
+                         phase=m.get("phase", (0.0, 0.0, 0.0)))
+    return jpabi.Uniform(m["n"], re=m["re"], u0=m["u0"], prec=prec, phase=m.get("phase", (0.0, 0.0, 0.0)))
+
+
+def levels(m):
+    return 3 if m["amr"] else 1
+
+
+def same(ens, alone):
+    for m, e, a in zip(MEMBERS, ens.members, alone):
+        for lvl in range(levels(m)):
+            assert np.array_equal(e.raw(lvl), a.raw(lvl)), (m, lvl)
+
+
+@pytest.mark.parametrize("prec", [jpabi.PREC_FP32, jpabi.PREC_FP16])
+def test_batched_members_match_single_contexts(prec):
+    ens = jpabi.Ensemble(MEMBERS, prec=prec, maxgb=16.0)
+    ens.init()
+    alone = [standalone(m, prec) for m in MEMBERS]
+    for a in alone:
+        a.init()
+    same(ens, alone)  # identical device code paths from the same parameters
+    ens.step(3)
+    for a in alone:
+        a.step(3)
+    same(ens, alone)
+
+
+def test_graph_replay_and_regrid_match():
+    prec = jpabi.PREC_FP32
+    ens = jpabi.Ensemble(MEMBERS, prec=prec, maxgb=16.0)
+    ens.init()
+    alone = [standalone(m, prec) for m in MEMBERS]
+    for a in alone:
+        a.init()
+    ens.use_graph(True)
+    for _ in range(3):
+        ens.step(4)
+        for a in alone:
+            a.step(4)
+        same(ens, alone)
+        ens.regrid()
+        for m, a in zip(MEMBERS, alone):
+            if m["amr"] and (m["l1"] == jpabi.MODE_SENSOR or m["l2"] == jpabi.MODE_SENSOR):
+                a.regrid()
+        same(ens, alone)
+
+
+def test_autotune_keeps_state():
+    ens = jpabi.Ensemble(MEMBERS, maxgb=16.0)
+    ens.init()
+    ens.step(2)
+    before = [[e.raw(l) for l in range(levels(m))] for m, e in zip(MEMBERS, ens.members)]
+    groups = ens.autotune()
+    assert len(groups) == 9 and all(g > 0 for g in groups)
+    after = [[e.raw(l) for l in range(levels(m))] for m, e in zip(MEMBERS, ens.members)]
+    for b, a in zip(before, after):
+        for x, y in zip(b, a):
+            assert np.array_equal(x, y)
+
+
+def test_ensemble_bench_counts_member_work():
+    ens = jpabi.Ensemble(MEMBERS, maxgb=16.0)
+    ens.init()
+    r = ens.bench(4, warmup=0, graph=True)
+    work = 0
+    for m, e in zip(MEMBERS, ens.members):
+        if m["amr"]:
+            c = e.counts()
+            work += (c["cnb0"] + 2 * c["nf1"] + 4 * c["nf2"]) * 512
+        else:
+            work += m["n"] ** 3
+    assert r["work"] == pytest.approx(4 * work)
