@@ -1,0 +1,75 @@
+"""Physics gates of the reference (src/tgv3d_uniform.cu:197-202; src/tgv3d_amr.cu:899-906;
+docs/validation.md:12-14) on the GPU path: the uniform TGV breaks down with a finite enstrophy peak,
+monotonic energy and held mass; the static gold (every level refined) reproduces the uniform 4 N0 run
+(enstrophy-peak time within 0.3 t*, E/E0 within 2 %)."""
+import math
+
+import numpy as np
+import pytest
+
+import jpabi
+
+pytestmark = pytest.mark.gpu
+
+
+def curve_uniform(n, re, tstar, every):
+    u = jpabi.Uniform(n, re=re)
+    u.init()
+    kf = u.derived()["kf"]
+    steps = int(tstar / (0.05 * kf) + 0.5)
+    out, s = [], 0
+    while True:
+        out.append((s * 0.05 * kf, *u.diag()))
+        if s >= steps:
+            break
+        k = min(every, steps - s)
+        u.step(k)
+        s += k
+    return np.array(out)
+
+
+def curve_amr(n0, re, tstar, every):
+    a = jpabi.Amr(n0, re=re, l1=jpabi.MODE_ALL, l2=jpabi.MODE_ALL)
+    a.init()
+    a.use_graph(True)
+    kf = a.derived()["kf"]
+    steps = int(tstar / (0.05 * kf) + 0.5)
+    out, s = [], 0
+    while True:
+        out.append((s * 0.05 * kf, *a.diag()))
+        if s >= steps:
+            break
+        k = min(every, steps - s)
+        a.step(k)
+        s += k
+    return np.array(out)
+
+
+def gate(c):
+    t, E, ens, m = c.T
+    ip = int(np.argmax(ens))
+    mono = bool(np.all(E[1:] <= E[:-1] * 1.0000001))
+    drift = float(np.max(np.abs(m - m[0]) / m[0]))
+    return t[ip], mono, drift
+
+
+def test_uniform_tgv_gate():
+    c = curve_uniform(128, 1600.0, 12.0, 100)
+    tp, mono, drift = gate(c)
+    assert math.isfinite(c[:, 2].max())
+    assert 4.0 < tp < 10.0, tp
+    assert mono and drift < 5e-4
+
+
+def test_static_gold_reproduces_uniform_4n():
+    n0, re = 32, 800.0
+    a = curve_amr(n0, re, 12.0, 25)  # coarse steps; t* per coarse step = 0.05 * 2 pi / n0
+    u = curve_uniform(4 * n0, re, 12.0, 100)
+    ta, mono_a, drift_a = gate(a)
+    tu, mono_u, _ = gate(u)
+    assert abs(ta - tu) <= 0.3, (ta, tu)
+    assert mono_a and drift_a < 5e-4
+    # E/E0 curves within 2 % at common times
+    ea = np.interp(u[:, 0], a[:, 0], a[:, 1] / a[0, 1])
+    eu = u[:, 1] / u[0, 1]
+    assert np.max(np.abs(ea - eu) / eu) < 0.02
