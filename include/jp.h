@@ -183,6 +183,11 @@ int jp_step(jp_ctx* c, long n);
  * (src/tgv3d_amr.cu:872-881, 896). */
 int jp_use_graph(jp_ctx* c, int on);
 
+/* Launch variant of the dense-lattice step (not in the reference; a tuning knob): bits 0-1 select the
+ * workgroup size (256, 128, 512, 1024 cells), bit 2 cached instead of streaming population loads, bit 3
+ * cached instead of streaming stores. Every variant computes the same result. */
+int jp_set_variant(jp_ctx* c, int variant);
+
 /* Energy <|u|^2/2>, enstrophy <|omega|^2/2>, mass <rho>, reduced on the device in FP64 with a fixed
  * order; 24 bytes cross to the host (src/tgv3d_uniform.cu:170-183; src/tgv3d_amr.cu:882-885). */
 int jp_diag(jp_ctx* c, double out[3]);

@@ -223,15 +223,36 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
     HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
     if (int r = scan_l2(c)) return r;
-    if (a.nf2 > a.cap2)
-      return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
-                                          std::to_string(a.cap2) + " (threshold ties); raise --maxgb or lower --frac2");
+    // Ties at the threshold can exceed the level-2 pool (the reference writes out of bounds there): the
+    // pool grows when the budget allows; the refinement target cap_nref1 stays as measured at init.
+    bool grown = false;
+    if (a.nf2 > a.cap2) {
+      int cn = std::max(a.nref1, (a.cap2 / 8) * 3 / 2);
+      while (cn > a.nref1 && !pool_fits(a, a.cap1, 8L * cn)) cn--;
+      if (!pool_fits(a, a.cap1, 8L * cn))
+        return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
+                                            std::to_string(a.cap2) + " and the budget (threshold ties); raise --maxgb");
+      HCK(c, hipStreamSynchronize(c->st));
+      void* keep = a.f2a;  // old level-2 data, read by the migration below
+      for (void* p : {a.f2b, a.gh2, static_cast<void*>(a.l2pl1), static_cast<void*>(a.l2oct), static_cast<void*>(a.nb2)})
+        ctx_free(c, p);
+      a.cap2 = 8 * cn;
+      if (int r = alloc_l2_pool(c)) return r;
+      ctx_free(c, a.f2a);  // alloc_l2_pool made both buffers; the old data stays in `keep`
+      a.f2a = keep;
+      grown = true;
+    }
     if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
     if (a.nref1) HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
     if (a.nf2) {
       HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
                             a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
       std::swap(a.f2a, a.f2b);
+    }
+    if (grown) {  // the other buffer still has the old size
+      HCK(c, hipStreamSynchronize(c->st));
+      ctx_free(c, a.f2b);
+      if (int r = ctx_alloc(c, &a.f2b, size_t(a.cap2) * Q * MB3 * prec_bytes(c->prec), "f2b")) return r;
     }
   }
   if (int r = finish_tables(c)) return r;

@@ -272,6 +272,14 @@ int jp_step(jp_ctx* c, long n) {
   return JP_OK;
 }
 
+int jp_set_variant(jp_ctx* c, int variant) {
+  if (!c) return JP_ERR_ARG;
+  if (variant < 0 || variant > 15) return fail(c, JP_ERR_ARG, "variant must lie in [0, 15]");
+  c->variant = variant;
+  drop_graph(c);
+  return JP_OK;
+}
+
 int jp_use_graph(jp_ctx* c, int on) {
   if (!c) return JP_ERR_ARG;
   c->use_graph = on != 0;

@@ -25,7 +25,8 @@ __device__ inline Nbr neighbours(uint32_t x, uint32_t y, uint32_t z, uint32_t N)
 }
 
 // Fused pull stream-collide (BGK), src/tgv3d_uniform.cu:41-61, one lattice cell per lane.
-template <class S>
+// kNtLoad / kNtStore select streaming (non-temporal) cache policy for the population reads / writes.
+template <class S, bool kNtLoad = true, bool kNtStore = true>
 __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                        const Dense& g, typename S::real omega, float enc, float dec, uint32_t cell) {
   using R = typename S::real;
@@ -35,7 +36,9 @@ __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, t
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const uint32_t src = n.zs[CZ(q) + 1] + n.ys[CY(q) + 1] + n.xs[CX(q) + 1];
-    fin[q] = S::template load<q>(__builtin_nontemporal_load(fa + uint64_t(q) * g.plane + src), dec);
+    const typename S::raw* p = fa + uint64_t(q) * g.plane + src;
+    if constexpr (kNtLoad) fin[q] = S::template load<q>(__builtin_nontemporal_load(p), dec);
+    else fin[q] = S::template load<q>(*p, dec);
   });
   const Mom<R> m = moments(fin);
   const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
@@ -43,7 +46,9 @@ __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, t
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const R fo = fin[q] - omega * (fin[q] - feq<q>(e));
-    __builtin_nontemporal_store(S::template store<q>(fo, enc), fb + uint64_t(q) * g.plane + cell);
+    typename S::raw* p = fb + uint64_t(q) * g.plane + cell;
+    if constexpr (kNtStore) __builtin_nontemporal_store(S::template store<q>(fo, enc), p);
+    else *p = S::template store<q>(fo, enc);
   });
 }
 

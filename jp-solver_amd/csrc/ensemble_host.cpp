@@ -29,7 +29,6 @@ struct EnsState {
   long steps = 0;
 };
 
-static size_t real_bytes(int prec) { return prec == JP_PREC_FP64 ? 8 : 4; }
 
 static MemberDesc describe(const jp_ctx* m) {
   MemberDesc d;

@@ -32,11 +32,11 @@ __global__ __launch_bounds__(BS) void k_dense_init(typename S::raw* __restrict__
 }
 
 // Fused pull stream-collide over the whole lattice, one cell per lane.
-template <class S, int BS>
+template <class S, int BS, bool kNtLoad, bool kNtStore>
 __global__ __launch_bounds__(BS) void k_dense_step(const typename S::raw* __restrict__ fa,
                                                    typename S::raw* __restrict__ fb, Dense g,
                                                    typename S::real omega, float enc, float dec) {
-  dense_step_cell<S>(fa, fb, g, omega, enc, dec, blockIdx.x * BS + threadIdx.x);
+  dense_step_cell<S, kNtLoad, kNtStore>(fa, fb, g, omega, enc, dec, blockIdx.x * BS + threadIdx.x);
 }
 
 // Macroscopic field, src/tgv3d_uniform.cu:80-90.
@@ -177,21 +177,41 @@ hipError_t dense_init(int prec, void* f, const Dense& g, double u0, double kf, c
   return hipErrorInvalidValue;
 }
 
-template <class S> static hipError_t step_t(const void* fa, void* fb, const Dense& g, double omega, float enc,
-                                            float dec, hipStream_t st) {
+template <class S, int BS, bool NL, bool NS>
+static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, hipStream_t st) {
   using R = typename S::real;
   const uint64_t cells = uint64_t(g.N) * g.N * g.N;
-  hipLaunchKernelGGL((k_dense_step<S, kBS>), dim3(uint32_t(cells / kBS)), dim3(kBS), 0, st,
+  hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
                      static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
   return hipGetLastError();
 }
-hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int,
+// variant: bits 0-1 workgroup size (256, 128, 512, 1024); bit 2 cached loads; bit 3 cached stores.
+template <class S> static hipError_t step_t(const void* fa, void* fb, const Dense& g, double omega, float enc,
+                                            float dec, int v, hipStream_t st) {
+  const uint64_t cells = uint64_t(g.N) * g.N * g.N;
+  int bs = (v & 3) == 1 ? 128 : (v & 3) == 2 ? 512 : (v & 3) == 3 ? 1024 : 256;
+  if (cells % bs) v &= ~3;  // N^3 is a multiple of 512; 1024 needs an even tile count
+  const bool nl = !(v & 4), ns = !(v & 8);
+#define JP_STEP_V(BS_)                                                              \
+  if (nl && ns) return step_v<S, BS_, true, true>(fa, fb, g, omega, enc, dec, st);   \
+  if (nl) return step_v<S, BS_, true, false>(fa, fb, g, omega, enc, dec, st);        \
+  if (ns) return step_v<S, BS_, false, true>(fa, fb, g, omega, enc, dec, st);        \
+  return step_v<S, BS_, false, false>(fa, fb, g, omega, enc, dec, st);
+  switch (v & 3) {
+    case 1: { JP_STEP_V(128) }
+    case 2: { JP_STEP_V(512) }
+    case 3: { JP_STEP_V(1024) }
+    default: { JP_STEP_V(256) }
+  }
+#undef JP_STEP_V
+}
+hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int variant,
                       hipStream_t st) {
   switch (prec) {
-    case 0: return step_t<FormF32>(fa, fb, g, omega, enc, dec, st);
-    case 1: return step_t<FormF16>(fa, fb, g, omega, enc, dec, st);
-    case 2: return step_t<FormBF16>(fa, fb, g, omega, enc, dec, st);
-    case 3: return step_t<FormF64>(fa, fb, g, omega, enc, dec, st);
+    case 0: return step_t<FormF32>(fa, fb, g, omega, enc, dec, variant, st);
+    case 1: return step_t<FormF16>(fa, fb, g, omega, enc, dec, variant, st);
+    case 2: return step_t<FormBF16>(fa, fb, g, omega, enc, dec, variant, st);
+    case 3: return step_t<FormF64>(fa, fb, g, omega, enc, dec, variant, st);
   }
   return hipErrorInvalidValue;
 }

@@ -101,7 +101,7 @@ def lib():
             "jp_regrid_with_sensor": (i, [vp, vp, vp]),
             "jp_get_counts": (i, [vp, ctypes.POINTER(Counts)]),
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
-            "jp_use_graph": (i, [vp, i]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
+            "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
             "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -195,6 +195,9 @@ class _Ctx:
 
     def use_graph(self, on=True):
         self._check(lib().jp_use_graph(self.h, 1 if on else 0))
+
+    def set_variant(self, v):
+        self._check(lib().jp_set_variant(self.h, v))
 
     def sync(self):
         self._check(lib().jp_sync(self.h))

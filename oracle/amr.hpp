@@ -688,11 +688,26 @@ template <class S> struct Amr {
       nref1 = scan_count(ref1, nf1);
       nf2 = 8 * nref1;
       hw_nf2 = std::max(hw_nf2, long(nf2));
-      if (nf2 > cap2) overflow = true;
+      // Ties at the threshold can exceed the level-2 pool (the reference writes out of bounds there):
+      // the pool grows when the budget allows, the refinement target cap_nref1 stays as measured.
+      std::vector<raw> oldf2;
+      if (nf2 > cap2) {
+        int cn = std::max(nref1, (cap2 / 8) * 3 / 2);
+        while (cn > nref1 && !pool_fits(cap1, 8L * cn)) cn--;
+        if (!pool_fits(cap1, 8L * cn)) {
+          overflow = true;
+          return;
+        }
+        cap2 = 8 * cn;
+        oldf2 = f2a;
+        ensure_l2_pool();
+        f2a = oldf2;
+      }
       std::fill(fof1.begin(), fof1.end(), -1);
       if (nref1) scatter_L2(cap2);
       if (overflow) return;
       if (nf2) migrate_L2();
+      if (f2b.size() != f2a.size()) f2b.assign(f2a.size(), raw(0));
     }
     build_skins();
   }

@@ -208,3 +208,30 @@ def test_amr_bench_and_graph_replay():
     same_state(g, o)
     c = g.counts()
     assert r["work"] == pytest.approx(4 * (c["cnb0"] + 2 * c["nf1"] + 4 * c["nf2"]) * 512)
+
+
+def test_level2_pool_grows_on_tied_regrid():
+    """A regrid whose threshold ties exceed the level-2 pool grows the pool (the reference would write out
+    of bounds); tables and populations stay bit-exact with the oracle, which applies the same rule."""
+    n0 = 32
+    cnb0, ng1 = (n0 // 8) ** 3, (n0 // 4) ** 3
+    rng = np.random.default_rng(23)
+    s0 = rng.random(cnb0).astype(np.float32)
+    s1 = rng.random(ng1).astype(np.float32)
+    g, o = pair(n=n0, re=1600.0, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.5, frac2=0.05)
+    g.init(s0, s1)
+    o.init(s0, s1)
+    cap2 = g.counts()["cap2"]
+    copy_state(o, g)
+    g.step(2)
+    o.step(2)
+    s1 = np.ones(ng1, np.float32)  # every deep level-1 tile tied at the threshold
+    g.regrid(s0, s1)
+    assert o.regrid(s0, s1) == 0
+    c = g.counts()
+    assert c["nf2"] > cap2 and c["cap2"] >= c["nf2"]
+    same_tables(g, o)
+    same_state(g, o)
+    g.step(2)
+    o.step(2)
+    same_state(g, o)

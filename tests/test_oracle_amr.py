@@ -146,3 +146,23 @@ def test_overflow_is_reported():
     cnb0 = (n0 // 8) ** 3
     a = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.1, frac2=0.1)
     assert a.init(sen0=np.ones(cnb0, np.float32)) == 3  # all tied: every tile marked, above cap_nref0
+
+
+def test_level2_pool_grows_within_budget_and_overflows_beyond():
+    n0 = 32
+    cnb0, ng1 = (n0 // 8) ** 3, (n0 // 4) ** 3
+    rng = np.random.default_rng(23)
+    s0, s1 = rng.random(cnb0).astype(np.float32), rng.random(ng1).astype(np.float32)
+    a = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.5, frac2=0.05)
+    assert a.init(s0, s1) == 0
+    cap2 = a.counts()["cap2"]
+    assert a.regrid(s0, np.ones(ng1, np.float32)) == 0
+    c = a.counts()
+    assert c["nf2"] > cap2 and c["cap2"] >= c["nf2"]
+    check_table_invariants(a, n0)
+    # with a budget that only fits the initial pools, the same regrid is an overflow
+    PB = 19 * 4 * (2 * 512 + 1000) * 6 / 5
+    b = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.5, frac2=0.05,
+                budget_bytes=(a.counts()["cap1"] + cap2) * PB * 1.0001)
+    assert b.init(s0, s1) == 0
+    assert b.regrid(s0, np.ones(ng1, np.float32)) == 3
