@@ -184,8 +184,9 @@ int jp_step(jp_ctx* c, long n);
 int jp_use_graph(jp_ctx* c, int on);
 
 /* Launch variant of the dense-lattice step (not in the reference; a tuning knob): bits 0-1 select the
- * workgroup size (256, 128, 512, 1024 cells), bit 2 cached instead of streaming population loads, bit 3
- * cached instead of streaming stores. Every variant computes the same result. */
+ * workgroup size (256, 128, 512, 1024 lanes), bit 2 cached instead of streaming population loads, bit 3
+ * cached instead of streaming stores, bits 4-5 consecutive x-cells per lane (1, 2, 4, 8) read and written
+ * as one vector per direction. Every variant computes the same result. */
 int jp_set_variant(jp_ctx* c, int variant);
 
 /* Energy <|u|^2/2>, enstrophy <|omega|^2/2>, mass <rho>, reduced on the device in FP64 with a fixed

@@ -274,7 +274,7 @@ int jp_step(jp_ctx* c, long n) {
 
 int jp_set_variant(jp_ctx* c, int variant) {
   if (!c) return JP_ERR_ARG;
-  if (variant < 0 || variant > 15) return fail(c, JP_ERR_ARG, "variant must lie in [0, 15]");
+  if (variant < 0 || variant > 63) return fail(c, JP_ERR_ARG, "variant must lie in [0, 63]");
   c->variant = variant;
   drop_graph(c);
   return JP_OK;

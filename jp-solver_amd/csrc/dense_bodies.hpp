@@ -24,6 +24,80 @@ __device__ inline Nbr neighbours(uint32_t x, uint32_t y, uint32_t z, uint32_t N)
   return n;
 }
 
+// Fused pull stream-collide over V consecutive x-cells per lane: each direction is one aligned vector
+// load of the source row segment (a wave reads 64*V contiguous elements of a plane per instruction);
+// the x-shifted directions take the element that crosses the lane boundary from the neighbouring lane
+// by a cross-lane shuffle, and only lanes at a wave or row edge load it themselves (periodic wrap).
+// Each cell's arithmetic is exactly that of dense_step_cell.
+template <class T, int V> struct VecT {
+  typedef T type __attribute__((ext_vector_type(V)));
+};
+template <class S, int V, bool kNtLoad, bool kNtStore>
+__device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                      const Dense& g, typename S::real omega, float enc, float dec, uint32_t lane_id) {
+  using R = typename S::real;
+  using raw = typename S::raw;
+  using vec = typename VecT<raw, V>::type;
+  const uint32_t N = g.N, NN = N * N;
+  const uint32_t cell0 = lane_id * V;
+  const uint32_t row = fdiv(cell0, g.divN), x0 = cell0 - row * N, z = fdiv(row, g.divN), y = row - z * N;
+  const uint32_t ys[3] = {(y == N - 1 ? 0 : y + 1) * N, y * N, (y == 0 ? N - 1 : y - 1) * N};
+  const uint32_t zs[3] = {(z == N - 1 ? 0 : z + 1) * NN, z * NN, (z == 0 ? N - 1 : z - 1) * NN};
+  const uint32_t xl = x0 == 0 ? N - 1 : x0 - 1, xr = x0 + V == N ? 0 : x0 + V;
+  const int lane = threadIdx.x & 63;
+  const bool left_edge = lane == 0 || x0 == 0, right_edge = lane == 63 || x0 + V == N;
+  R f[Q][V];
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = CX(q);
+    const raw* base = fa + uint64_t(q) * g.plane + zs[CZ(q) + 1] + ys[CY(q) + 1];
+    const vec* vp = reinterpret_cast<const vec*>(base + x0);
+    vec w;
+    if constexpr (kNtLoad) w = __builtin_nontemporal_load(vp);
+    else w = *vp;
+    R v[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = S::template load<q>(w[j], dec);
+    if constexpr (cx == 0) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) f[q][j] = v[j];
+    } else if constexpr (cx > 0) {  // source x - 1
+      R l = __shfl_up(v[V - 1], 1, 64);
+      if (left_edge) l = S::template load<q>(base[xl], dec);
+      f[q][0] = l;
+#pragma unroll
+      for (int j = 1; j < V; ++j) f[q][j] = v[j - 1];
+    } else {  // source x + 1
+      R r = __shfl_down(v[0], 1, 64);
+      if (right_edge) r = S::template load<q>(base[xr], dec);
+#pragma unroll
+      for (int j = 0; j < V - 1; ++j) f[q][j] = v[j + 1];
+      f[q][V - 1] = r;
+    }
+  });
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    R fin[Q];
+    sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = f[decltype(qc)::value][j]; });
+    const Mom<R> m = moments(fin);
+    const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+    const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      f[q][j] = fin[q] - omega * (fin[q] - feq<q>(e));
+    });
+  }
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    vec w;
+#pragma unroll
+    for (int j = 0; j < V; ++j) w[j] = S::template store<q>(f[q][j], enc);
+    vec* p = reinterpret_cast<vec*>(fb + uint64_t(q) * g.plane + cell0);
+    if constexpr (kNtStore) __builtin_nontemporal_store(w, p);
+    else *p = w;
+  });
+}
+
 // Fused pull stream-collide (BGK), src/tgv3d_uniform.cu:41-61, one lattice cell per lane.
 // kNtLoad / kNtStore select streaming (non-temporal) cache policy for the population reads / writes.
 template <class S, bool kNtLoad = true, bool kNtStore = true>

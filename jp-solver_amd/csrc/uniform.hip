@@ -177,26 +177,35 @@ hipError_t dense_init(int prec, void* f, const Dense& g, double u0, double kf, c
   return hipErrorInvalidValue;
 }
 
-template <class S, int BS, bool NL, bool NS>
+// V consecutive x-cells per lane (V = 1: dense_step_cell)
+template <class S, int BS, int V, bool kNtLoad, bool kNtStore>
+__global__ __launch_bounds__(BS) void k_dense_step_vec(const typename S::raw* __restrict__ fa,
+                                                       typename S::raw* __restrict__ fb, Dense g,
+                                                       typename S::real omega, float enc, float dec) {
+  dense_step_vec<S, V, kNtLoad, kNtStore>(fa, fb, g, omega, enc, dec, blockIdx.x * BS + threadIdx.x);
+}
+
+template <class S, int BS, int V, bool NL, bool NS>
 static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, hipStream_t st) {
   using R = typename S::real;
   const uint64_t cells = uint64_t(g.N) * g.N * g.N;
-  hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
-                     static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
+  if constexpr (V == 1)
+    hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
+                       static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
+  else
+    hipLaunchKernelGGL((k_dense_step_vec<S, BS, V, NL, NS>), dim3(uint32_t(cells / (uint64_t(BS) * V))), dim3(BS), 0, st,
+                       static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
   return hipGetLastError();
 }
-// variant: bits 0-1 workgroup size (256, 128, 512, 1024); bit 2 cached loads; bit 3 cached stores.
-template <class S> static hipError_t step_t(const void* fa, void* fb, const Dense& g, double omega, float enc,
-                                            float dec, int v, hipStream_t st) {
-  const uint64_t cells = uint64_t(g.N) * g.N * g.N;
-  int bs = (v & 3) == 1 ? 128 : (v & 3) == 2 ? 512 : (v & 3) == 3 ? 1024 : 256;
-  if (cells % bs) v &= ~3;  // N^3 is a multiple of 512; 1024 needs an even tile count
+template <class S, int V>
+static hipError_t step_bs(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int v,
+                          hipStream_t st) {
   const bool nl = !(v & 4), ns = !(v & 8);
-#define JP_STEP_V(BS_)                                                              \
-  if (nl && ns) return step_v<S, BS_, true, true>(fa, fb, g, omega, enc, dec, st);   \
-  if (nl) return step_v<S, BS_, true, false>(fa, fb, g, omega, enc, dec, st);        \
-  if (ns) return step_v<S, BS_, false, true>(fa, fb, g, omega, enc, dec, st);        \
-  return step_v<S, BS_, false, false>(fa, fb, g, omega, enc, dec, st);
+#define JP_STEP_V(BS_)                                                                 \
+  if (nl && ns) return step_v<S, BS_, V, true, true>(fa, fb, g, omega, enc, dec, st);   \
+  if (nl) return step_v<S, BS_, V, true, false>(fa, fb, g, omega, enc, dec, st);        \
+  if (ns) return step_v<S, BS_, V, false, true>(fa, fb, g, omega, enc, dec, st);        \
+  return step_v<S, BS_, V, false, false>(fa, fb, g, omega, enc, dec, st);
   switch (v & 3) {
     case 1: { JP_STEP_V(128) }
     case 2: { JP_STEP_V(512) }
@@ -204,6 +213,26 @@ template <class S> static hipError_t step_t(const void* fa, void* fb, const Dens
     default: { JP_STEP_V(256) }
   }
 #undef JP_STEP_V
+}
+// variant: bits 0-1 workgroup size (256, 128, 512, 1024); bit 2 cached loads; bit 3 cached stores;
+// bits 4-5 cells per lane (1, 2, 4, 8). Shapes that do not tile N^3 fall back to smaller ones.
+template <class S> static hipError_t step_t(const void* fa, void* fb, const Dense& g, double omega, float enc,
+                                            float dec, int v, hipStream_t st) {
+  const uint64_t cells = uint64_t(g.N) * g.N * g.N;
+  int bs = (v & 3) == 1 ? 128 : (v & 3) == 2 ? 512 : (v & 3) == 3 ? 1024 : 256;
+  int V = 1 << ((v >> 4) & 3);
+  if (g.N % V) V = 1;
+  while (V > 1 && cells % (uint64_t(bs) * V)) V >>= 1;
+  if (cells % bs) {
+    v &= ~3;  // N^3 is a multiple of 512; 1024 needs an even tile count
+    bs = 256;
+  }
+  switch (V) {
+    case 2: return step_bs<S, 2>(fa, fb, g, omega, enc, dec, v, st);
+    case 4: return step_bs<S, 4>(fa, fb, g, omega, enc, dec, v, st);
+    case 8: return step_bs<S, 8>(fa, fb, g, omega, enc, dec, v, st);
+    default: return step_bs<S, 1>(fa, fb, g, omega, enc, dec, v, st);
+  }
 }
 hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int variant,
                       hipStream_t st) {
