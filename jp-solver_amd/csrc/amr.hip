@@ -20,6 +20,23 @@ __global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __rest
   __syncthreads();
   fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x);
 }
+// Last fine substep with the restriction into the parent level fused in (ptile/poct: the parent tile
+// and octant of each fine tile, i.e. l2pl1/l2oct or l1pcb/l1oct).
+template <class S, bool kToDense>
+__global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                                     const typename S::real* __restrict__ gh, const int* __restrict__ nb,
+                                                     typename S::real om, float enc, float dec, typename S::raw* __restrict__ cf,
+                                                     const int* __restrict__ ptile, const int* __restrict__ poct, AmrGeo G,
+                                                     float nqi, float cenc) {
+  __shared__ int snb[kNbr];
+  __shared__ typename S::real keep[Q * MB3];
+  const int b = blockIdx.x;
+  if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
+  __syncthreads();
+  fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x, keep);
+  restrict_tile_lds<S, kToDense>(keep, threadIdx.x, cf, ptile[b], poct[b], G, nqi, cenc);
+}
+
 template <class S, bool kDense>
 __global__ __launch_bounds__(512) void k_prep(const typename S::raw* __restrict__ f, typename S::real* __restrict__ pr,
                                               const int* __restrict__ list, AmrGeo G, float nqs, float dec) {
@@ -261,6 +278,20 @@ hipError_t amr_fine_step(int prec, int, const void* fa, void* fb, const void* gh
   if (ntiles <= 0) return hipSuccess;
   JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step<S>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh), nb,
                                        typename S::real(omega), enc, dec));
+  return hipGetLastError();
+}
+
+hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
+                                  const int* nb, int ntiles, double omega, float enc, float dec, void* cf, const int* ptile,
+                                  const int* poct, float nqi, float cenc, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  if (to_dense) {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step_r<S, true>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh),
+                                         nb, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G, nqi, cenc));
+  } else {
+    JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step_r<S, false>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh),
+                                         nb, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G, nqi, cenc));
+  }
   return hipGetLastError();
 }
 

@@ -47,15 +47,68 @@ template <class S> __device__ inline void load_dense(const typename S::raw* __re
 }
 
 // BGK collision of pulled populations, stored to tile b (src/tgv3d_amr.cu:268-273).
+// keep (optional, LDS [19][512]): receives each stored value as it reads back, for the fused restriction.
 template <class S> __device__ inline void collide_store(typename S::real* fin, typename S::real om,
-                                                        typename S::raw* __restrict__ fb, int b, int cell, float enc) {
+                                                        typename S::raw* __restrict__ fb, int b, int cell, float enc,
+                                                        typename S::real* keep = nullptr, float dec = 1.f) {
   using R = typename S::real;
   const Mom<R> m = moments(fin);
   const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
   const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    fb[tidx(b, q, cell)] = S::template store<q>(fin[q] - om * (fin[q] - feq<q>(e)), enc);
+    const typename S::raw v = S::template store<q>(fin[q] - om * (fin[q] - feq<q>(e)), enc);
+    fb[tidx(b, q, cell)] = v;
+    if (keep) keep[q * MB3 + cell] = S::template load<q>(v, dec);
+  });
+}
+
+// Restriction of one tile's 64 parent cells from the tile's just-stored populations held in LDS
+// (src/tgv3d_amr.cu:275-298, 358-381): the 8 children of a parent cell lie in this tile, so the average
+// needs no second pass over the fine level. The 512 lanes first sum (in the reference's child order)
+// 64 parents x 19 directions, then 64 lanes finish one parent cell each. Call with all lanes.
+template <class S, bool kToDense>
+__device__ inline void restrict_tile_lds(typename S::real* lds, int cell, typename S::raw* __restrict__ cf, int pcb,
+                                         int oct, const AmrGeo& G, float nqi, float cenc) {
+  using R = typename S::real;
+  __syncthreads();
+  const int p = cell & 63, part = cell >> 6;  // parent (pi, pj, pk) = (p & 3, (p >> 2) & 3, p >> 4)
+  const int c0 = ((2 * (p >> 4)) * MB + 2 * ((p >> 2) & 3)) * MB + 2 * (p & 3);
+  R s[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int q = part + 8 * u;
+    R acc = R(0);
+    if (q < Q) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc += R(0.125) * lds[q * MB3 + c0 + (c >> 2) * 64 + ((c >> 1) & 1) * 8 + (c & 1)];
+    }
+    s[u] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (part + 8 * u < Q) lds[(part + 8 * u) * 64 + p] = s[u];
+  __syncthreads();
+  if (cell >= 64) return;
+  R avg[Q], nq[Q];
+  sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = lds[decltype(qc)::value * 64 + p]; });
+  const Mom<R> m = moments(avg);
+  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    nq[q] = avg[q] - feq<q>(e);
+  });
+  regularize(nq);
+  const int lci = 4 * (oct & 1) + (p & 3), lcj = 4 * ((oct >> 1) & 1) + ((p >> 2) & 3), lck = 4 * (oct >> 2) + (p >> 4);
+  const int pc = (lck * MB + lcj) * MB + lci;
+  const uint32_t di = kToDense ? dense_of_tile(G, pcb, pc) : 0;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
+    if constexpr (kToDense) cf[uint64_t(q) * G.g0.plane + di] = v;
+    else cf[tidx(pcb, q, pc)] = v;
   });
 }
 
@@ -66,7 +119,7 @@ template <class S> __device__ inline void collide_store(typename S::real* fin, t
 template <class S>
 __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
-                                      float enc, float dec, int b, int cell) {
+                                      float enc, float dec, int b, int cell, typename S::real* keep = nullptr) {
   using R = typename S::real;
   const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
   const R* __restrict__ g = gh + size_t(b) * kGhostSlots;
@@ -86,7 +139,7 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
       else fin[q] = g[ghost_slot(q, i, j, k)];
     }
   });
-  collide_store<S>(fin, om, fb, b, cell, enc);
+  collide_store<S>(fin, om, fb, b, cell, enc, keep, dec);
 }
 
 // prep = feq + nqs * regularize(f - feq) (src/tgv3d_amr.cu:160-171), listed tiles only.

@@ -70,7 +70,16 @@ static int alloc_l2_pool(jp_ctx* c) {
   if (int rc = ctx_alloc_t(c, &a.l2pl1, a.cap2, "l2pl1")) return rc;
   if (int rc = ctx_alloc_t(c, &a.l2oct, a.cap2, "l2oct")) return rc;
   if (int rc = ctx_alloc_t(c, &a.nb2, size_t(a.cap2) * kNbr, "nb2")) return rc;
+  a.cap2_alloc = a.cap2;
   return JP_OK;
+}
+static void free_l2_pool(jp_ctx* c) {
+  AmrState& a = c->amr;
+  for (void** p : {&a.f2a, &a.f2b, &a.gh2, reinterpret_cast<void**>(&a.l2pl1), reinterpret_cast<void**>(&a.l2oct),
+                   reinterpret_cast<void**>(&a.nb2)}) {
+    ctx_free(c, *p);
+    *p = nullptr;
+  }
 }
 
 // skins (src/tgv3d_amr.cu:758-762) and the 27-neighbour tables of both fine levels
@@ -90,8 +99,39 @@ static int finish_tables(jp_ctx* c) {
   return JP_OK;
 }
 
+static int alloc_l1_pool(jp_ctx* c, bool with_f1a) {
+  AmrState& a = c->amr;
+  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
+  if (with_f1a)
+    if (int rc = ctx_alloc(c, &a.f1a, size_t(a.cap1) * tile * e, "f1a")) return rc;
+  if (int rc = ctx_alloc(c, &a.f1b, size_t(a.cap1) * tile * e, "f1b")) return rc;
+  if (int rc = ctx_alloc(c, &a.prep1, size_t(a.cap1) * tile * r, "prep1")) return rc;
+  if (int rc = ctx_alloc(c, &a.gh1, size_t(a.cap1) * kGhostSlots * r, "gh1")) return rc;
+  return JP_OK;
+}
+
+// Ties at the level-0 threshold can exceed the level-1 pool (the reference writes out of bounds there;
+// a symmetric TGV ties every tile at small N0). The pool grows when the budget allows; the refinement
+// target cap_nref0 stays as the reference sizes it. keep_f1a: a regrid's migration still reads the old
+// level-1 data, so f1a is kept at its old size and replaced after the migration.
+static int grow_l1(jp_ctx* c, bool keep_f1a) {
+  AmrState& a = c->amr;
+  const long c2 = a.f2a ? a.cap2 : 0;
+  int cn = std::min(a.cnb0, std::max(a.nref0, (a.cap1 / 8) * 3 / 2));
+  while (cn > a.nref0 && !pool_fits(a, 8L * cn, c2)) cn--;
+  if (!pool_fits(a, 8L * cn, c2))
+    return fail(c, JP_ERR_CAPACITY, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
+                                        std::to_string(a.cap1) + " and the budget (threshold ties); raise --maxgb");
+  HCK(c, hipStreamSynchronize(c->st));
+  for (void* p : {a.f1b, a.prep1, a.gh1}) ctx_free(c, p);
+  if (!keep_f1a) ctx_free(c, a.f1a);
+  a.cap1 = 8 * cn;
+  a.f1_stale = keep_f1a;
+  return alloc_l1_pool(c, !keep_f1a);
+}
+
 // level-1 marks -> scan -> counts -> tables (common to init and regrid). ref0 is already marked.
-static int build_l1_tables(jp_ctx* c) {
+static int build_l1_tables(jp_ctx* c, bool keep_f1a) {
   AmrState& a = c->amr;
   HCK(c, rg_scan(a.ref0, a.cnb0, a.rank, a.dcnt, CNT_NREF0, c->st));
   if (int r = read_counts(c)) return r;
@@ -99,8 +139,7 @@ static int build_l1_tables(jp_ctx* c) {
   a.nf1 = 8 * a.nref0;
   a.hw_nf1 = std::max(a.hw_nf1, long(a.nf1));
   if (a.nf1 > a.cap1)
-    return fail(c, JP_ERR_CAPACITY, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
-                                        std::to_string(a.cap1) + " (threshold ties); raise --maxgb or lower --frac1");
+    if (int r = grow_l1(c, keep_f1a)) return r;
   if (int r = fill(c, a.l1grid, -1, size_t(a.NG1))) return r;
   if (int r = fill(c, a.fof0, -1, size_t(a.cnb0))) return r;
   if (a.nref0)
@@ -155,7 +194,7 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
     HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
   }
-  if (int r = build_l1_tables(c)) return r;
+  if (int r = build_l1_tables(c, false)) return r;
   if (a.nf1) HCK(c, amr_seed_l1(c->prec, G, c->c0a, a.f1a, a.l1pcb, a.l1oct, a.nf1, float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
   a.nref1 = a.nf2 = 0;
   if (a.nf1) {
@@ -176,6 +215,10 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   } else {
     a.cap_nref1 = 1;
     a.cap2 = 8;
+  }
+  if (a.f2a && a.cap2 > a.cap2_alloc) {  // a re-initialisation asks for more level-2 tiles than are held
+    HCK(c, hipStreamSynchronize(c->st));
+    free_l2_pool(c);
   }
   if (!a.f2a)
     if (int r = alloc_l2_pool(c)) return r;
@@ -210,11 +253,17 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
   HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
   HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
-  if (int r = build_l1_tables(c)) return r;
+  if (int r = build_l1_tables(c, true)) return r;
   if (a.nf1) {
     HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, a.nf1, float(c->d.nqs01),
                           dec(c, 0), enc(c, 1), c->st));
     std::swap(a.f1a, a.f1b);
+  }
+  if (a.f1_stale) {  // the pool grew: the buffer that held the old level-1 data still has the old size
+    HCK(c, hipStreamSynchronize(c->st));
+    ctx_free(c, a.f1b);
+    if (int r = ctx_alloc(c, &a.f1b, size_t(a.cap1) * Q * MB3 * prec_bytes(c->prec), "f1b")) return r;
+    a.f1_stale = false;
   }
   a.nref1 = a.nf2 = 0;
   if (a.nf1) {
@@ -278,18 +327,23 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
         HCK(c, amr_prep(p, 1, G, a.f1a, a.prep1, a.skin1, a.n_skin1, float(c->d.nqs12), dec(c, 1), st));
         HCK(c, amr_fill_l2(p, G, a.prep1, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.slots, a.nf2, st));
       }
-      for (int s2 = 0; s2 < 2; ++s2) {
+      // the second level-2 substep carries the restriction into level 1 in its epilogue
+      HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+      std::swap(a.f2a, a.f2b);
+      if (a.skiprestr)
         HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
-        std::swap(a.f2a, a.f2b);
-      }
-      if (!a.skiprestr)
-        HCK(c, amr_restrict(p, false, G, a.f1a, a.f2a, a.rl1, a.fof1, a.nref1, float(c->d.nqi21), enc(c, 1), dec(c, 2), st));
+      else
+        HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2),
+                                      a.f1a, a.l2pl1, a.l2oct, float(c->d.nqi21), enc(c, 1), st));
+      std::swap(a.f2a, a.f2b);
     }
-    HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+    if (s1 == 1 && !a.skiprestr)  // the second level-1 substep restricts into level 0
+      HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
+                                    a.l1pcb, a.l1oct, float(c->d.nqi10), enc(c, 0), st));
+    else
+      HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
     std::swap(a.f1a, a.f1b);
   }
-  if (!a.skiprestr)
-    HCK(c, amr_restrict(p, true, G, c->c0a, a.f1a, a.rl0, a.fof0, a.nref0, float(c->d.nqi10), enc(c, 0), dec(c, 1), st));
   return JP_OK;
 }
 
@@ -350,32 +404,31 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
   a.cap_nref1 = INT_MAX;
   // pools
   c->g0 = make_dense(N, 0);
-  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
+  c->variant = default_variant(c->prec);
+  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec);
   const size_t dense_b = size_t(Q) * c->g0.plane;
   if (int rc = ctx_alloc(c, &c->c0a, dense_b * e, "c0a")) return rc;
   if (int rc = ctx_alloc(c, &c->c0b, dense_b * e, "c0b")) return rc;
   if (int rc = ctx_alloc(c, &a.prep0, dense_b * r, "prep0")) return rc;
-  if (int rc = ctx_alloc(c, &a.f1a, size_t(a.cap1) * tile * e, "f1a")) return rc;
-  if (int rc = ctx_alloc(c, &a.f1b, size_t(a.cap1) * tile * e, "f1b")) return rc;
-  if (int rc = ctx_alloc(c, &a.prep1, size_t(a.cap1) * tile * r, "prep1")) return rc;
-  if (int rc = ctx_alloc(c, &a.gh1, size_t(a.cap1) * kGhostSlots * r, "gh1")) return rc;
-  const size_t n01 = size_t(std::max(a.cnb0, a.cap1));
+  if (int rc = alloc_l1_pool(c, true)) return rc;
+  // level-1 tables are sized for full refinement (8 cnb0 tiles) so that only the pools ever grow
+  const size_t n1 = size_t(8) * a.cnb0, n01 = n1;
   struct T {
     int** p;
     size_t n;
     const char* w;
   } tabs[] = {{&a.ref0, size_t(a.cnb0), "ref0"},     {&a.fof0, size_t(a.cnb0), "fof0"},     {&a.o_ref0, size_t(a.cnb0), "o_ref0"},
-              {&a.o_fof0, size_t(a.cnb0), "o_fof0"}, {&a.skin0, size_t(a.cnb0), "skin0"},   {&a.l1pcb, size_t(a.cap1), "l1pcb"},
-              {&a.l1oct, size_t(a.cap1), "l1oct"},   {&a.rl0, size_t(a.cap1), "rl0"},       {&a.f1pos, size_t(a.cap1), "f1pos"},
-              {&a.ref1, size_t(a.cap1), "ref1"},     {&a.fof1, size_t(a.cap1), "fof1"},     {&a.o_ref1, size_t(a.cap1), "o_ref1"},
-              {&a.o_fof1, size_t(a.cap1), "o_fof1"}, {&a.rl1, size_t(a.cap1), "rl1"},       {&a.deep, size_t(a.cap1), "deep"},
-              {&a.skin1, size_t(a.cap1), "skin1"},   {&a.l1grid, size_t(a.NG1), "l1grid"},  {&a.o_l1grid, size_t(a.NG1), "o_l1grid"},
-              {&a.rank, n01, "rank"},                {&a.flags, n01, "flags"},               {&a.nb1, size_t(a.cap1) * kNbr, "nb1"},
+              {&a.o_fof0, size_t(a.cnb0), "o_fof0"}, {&a.skin0, size_t(a.cnb0), "skin0"},   {&a.l1pcb, n1, "l1pcb"},
+              {&a.l1oct, n1, "l1oct"},   {&a.rl0, n1, "rl0"},       {&a.f1pos, n1, "f1pos"},
+              {&a.ref1, n1, "ref1"},     {&a.fof1, n1, "fof1"},     {&a.o_ref1, n1, "o_ref1"},
+              {&a.o_fof1, n1, "o_fof1"}, {&a.rl1, n1, "rl1"},       {&a.deep, n1, "deep"},
+              {&a.skin1, n1, "skin1"},   {&a.l1grid, size_t(a.NG1), "l1grid"},  {&a.o_l1grid, size_t(a.NG1), "o_l1grid"},
+              {&a.rank, n01, "rank"},                {&a.flags, n01, "flags"},               {&a.nb1, n1 * kNbr, "nb1"},
               {&a.dcnt, size_t(CNT_N), "counts"}};
   for (auto& t : tabs)
     if (int rc = ctx_alloc_t(c, t.p, t.n, t.w)) return rc;
   if (int rc = ctx_alloc_t(c, &a.sen0, size_t(a.cnb0), "sen0")) return rc;
-  if (int rc = ctx_alloc_t(c, &a.sen1, size_t(a.cap1), "sen1")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.sen1, n1, "sen1")) return rc;
   if (int rc = ctx_alloc_t(c, &a.sen1pos, size_t(a.NG1), "sen1 by position")) return rc;
   if (int rc = ctx_alloc_t(c, &a.thr, 4, "threshold")) return rc;
   if (int rc = ctx_alloc_t(c, &a.slots, size_t(kGhostSlots), "ghost fill table")) return rc;

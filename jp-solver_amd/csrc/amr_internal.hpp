@@ -46,6 +46,10 @@ struct AmrGeo {
 // ---- amr.hip ----
 hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const void* gh, const int* nb, int ntiles,
                          double omega, float enc, float dec, hipStream_t st);
+// fine step + restriction into the parent level (dense level 0 when to_dense)
+hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
+                                  const int* nb, int ntiles, double omega, float enc, float dec, void* cf, const int* ptile,
+                                  const int* poct, float nqi, float cenc, hipStream_t st);
 // prep = feq + nqs * regularize(f - feq) over the listed tiles; level 0 reads/writes dense planes
 hipError_t amr_prep(int prec, int level, const AmrGeo& G, const void* f, void* prep, const int* list, int n,
                     float nqs, float dec, hipStream_t st);

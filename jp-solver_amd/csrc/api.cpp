@@ -48,6 +48,10 @@ void ctx_free(jp_ctx* c, void* p) {
 }
 
 double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 / u0))); }
+// Dense-step launch variant per storage form, from the 512^3 sweep on MI355X: FP32 8 cells per lane in
+// 512-lane groups with cached loads and stores; 16-bit forms one cell per lane, cached loads, streaming
+// stores; FP64 one cell per lane, cached loads and stores.
+int default_variant(int prec) { return prec == 0 ? 62 : (prec == 3 ? 12 : 6); }
 
 int drop_graph(jp_ctx* c) {
   if (c->gexec) {
@@ -119,7 +123,8 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
   c->d.tau0 = 3.0 * c->d.nu + 0.5;
   c->d.omega0 = (p->prec == JP_PREC_FP64) ? 1.0 / c->d.tau0 : double(float(1.0 / c->d.tau0));
   c->d.kf = 2.0 * M_PI / double(N);
-  c->d.cells = long(N) * N * N;
+  c->g0 = make_dense(N, 0);
+  c->variant = default_variant(c->prec);
   const double s = (p->prec == JP_PREC_FP16 || p->prec == JP_PREC_BF16) ? (p->scale > 0 ? p->scale : default_scale(p->u0)) : 1.0;
   if (s != std::exp2(std::round(std::log2(s)))) return fail(c, JP_ERR_ARG, "scale must be a power of two");
   c->d.scale[0] = c->d.scale[1] = c->d.scale[2] = s;

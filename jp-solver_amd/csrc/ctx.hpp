@@ -21,6 +21,8 @@ struct AmrState {
   int cap_nref0 = 0, cap1 = 0, cap_nref1 = 0, cap2 = 0;
   long hw_nf1 = 0, hw_nf2 = 0;
   int nref0 = 0, nf1 = 0, nref1 = 0, nf2 = 0, n_skin0 = 0, n_skin1 = 0;
+  bool f1_stale = false;
+  int cap2_alloc = 0;  // level-2 tiles the allocated pool holds  // f1b still has a pre-growth size (until the regrid's migration completes)
   void *f1a = nullptr, *f1b = nullptr, *f2a = nullptr, *f2b = nullptr;  // stored populations
   void *prep0 = nullptr, *prep1 = nullptr, *gh1 = nullptr, *gh2 = nullptr;  // working-type values
   int *ref0 = nullptr, *fof0 = nullptr, *rl0 = nullptr, *l1pcb = nullptr, *l1oct = nullptr, *f1pos = nullptr;
@@ -68,6 +70,7 @@ template <class T> int ctx_alloc_t(jp_ctx* c, T** p, size_t count, const char* w
 void ctx_free(jp_ctx* c, void* p);
 int drop_graph(jp_ctx* c);
 double default_scale(double u0);
+int default_variant(int prec);
 // AMR orchestration (amr_host.cpp)
 int amr_init(jp_ctx* c);
 int amr_base_step(jp_ctx* c, hipStream_t st);

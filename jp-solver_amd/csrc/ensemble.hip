@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void kb_dense_step(const MemberDesc* __restric
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
-    dense_step_cell<S>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
+    dense_step_cell<S, false, true>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
                        uint32_t(w.y) * 256 + threadIdx.x);
   }
 }

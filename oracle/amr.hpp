@@ -71,30 +71,47 @@ template <class S> struct Amr {
     c0b.assign(nc, raw(0));
     prep0.assign(nc, R(0));
     f1a.assign(size_t(cap1) * Q * MB3, raw(0));
-    f1b = f1a;
-    prep1.assign(size_t(cap1) * Q * MB3, R(0));
-    gh1.assign(size_t(cap1) * Q * EB3, R(0));
+    l1_pool(false);
+    const int n1 = 8 * cnb0;  // level-1 tables are sized for full refinement; only the pools grow
     ref0.assign(cnb0, 0);
     fof0.assign(cnb0, -1);
     o_ref0 = ref0;
     o_fof0 = fof0;
-    l1pcb.assign(cap1, 0);
-    l1oct.assign(cap1, 0);
-    rl0.assign(cap1, 0);
-    f1pos.assign(cap1, 0);
+    l1pcb.assign(n1, 0);
+    l1oct.assign(n1, 0);
+    rl0.assign(n1, 0);
+    f1pos.assign(n1, 0);
     l1grid.assign(NG1, -1);
     o_l1grid = l1grid;
-    ref1.assign(cap1, 0);
-    fof1.assign(cap1, -1);
+    ref1.assign(n1, 0);
+    fof1.assign(n1, -1);
     o_ref1 = ref1;
     o_fof1 = fof1;
-    rl1.assign(cap1, 0);
+    rl1.assign(n1, 0);
     sen0.assign(cnb0, 0.f);
-    sen1.assign(cap1, 0.f);
+    sen1.assign(n1, 0.f);
     skin0.assign(cnb0, 0);
-    skin1.assign(cap1, 0);
-    rank.assign(std::max(cnb0, cap1), 0);
-    deep.assign(cap1, 0);
+    skin1.assign(n1, 0);
+    rank.assign(n1, 0);
+    deep.assign(n1, 0);
+  }
+  void l1_pool(bool with_f1a) {
+    if (with_f1a) f1a.assign(size_t(cap1) * Q * MB3, raw(0));
+    f1b.assign(size_t(cap1) * Q * MB3, raw(0));
+    prep1.assign(size_t(cap1) * Q * MB3, R(0));
+    gh1.assign(size_t(cap1) * Q * EB3, R(0));
+  }
+  // Ties at the level-0 threshold can exceed the level-1 pool (the reference writes out of bounds
+  // there; a symmetric TGV ties every tile at small N0): the pool grows to max(nref0, 1.5 x) tiles
+  // when the budget allows (never beyond full refinement); the target cap_nref0 stays. keep_f1a: a
+  // regrid still migrates from the old level-1 data.
+  bool grow_L1(bool keep_f1a) {
+    int cn = std::min(cnb0, std::max(nref0, (cap1 / 8) * 3 / 2));
+    while (cn > nref0 && !pool_fits(8L * cn, cap2)) cn--;
+    if (!pool_fits(8L * cn, cap2)) return false;
+    cap1 = 8 * cn;
+    l1_pool(!keep_f1a);
+    return true;
   }
   bool pool_fits(long c1, long c2) const {
     const double PB = double(Q) * 4 * (2 * MB3 + EB3);
@@ -595,7 +612,10 @@ template <class S> struct Amr {
     nref0 = scan_count(ref0, cnb0);
     nf1 = 8 * nref0;
     hw_nf1 = std::max(hw_nf1, long(nf1));
-    if (nf1 > cap1) overflow = true;
+    if (nf1 > cap1 && !grow_L1(false)) {
+      overflow = true;
+      return;
+    }
     std::fill(l1grid.begin(), l1grid.end(), -1);
     std::fill(fof0.begin(), fof0.end(), -1);
     if (nref0) scatter_L1();
@@ -662,12 +682,16 @@ template <class S> struct Amr {
     nref0 = scan_count(ref0, cnb0);
     nf1 = 8 * nref0;
     hw_nf1 = std::max(hw_nf1, long(nf1));
-    if (nf1 > cap1) overflow = true;
+    if (nf1 > cap1 && !grow_L1(true)) {
+      overflow = true;
+      return;
+    }
     std::fill(l1grid.begin(), l1grid.end(), -1);
     std::fill(fof0.begin(), fof0.end(), -1);
     if (nref0) scatter_L1();
     if (overflow) return;
     if (nf1) migrate_L1();
+    if (f1b.size() != f1a.size()) f1b.assign(f1a.size(), raw(0));
     nref1 = 0;
     nf2 = 0;
     if (nf1) {

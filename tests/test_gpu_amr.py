@@ -189,13 +189,50 @@ def test_dynamic_sensor_run_tables_follow_oracle():
         same_state(g, o)
 
 
+PB = 19 * 4 * (2 * 512 + 1000) * 6 / 5  # pool bytes per fine tile as the budget counts them
+
+
 def test_capacity_overflow_is_an_error_not_a_write():
     n0 = 24
     cnb0 = (n0 // 8) ** 3
-    g = jpabi.Amr(n=n0, re=1600.0, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.1, frac2=0.1, maxgb=64.0)
+    cap_nref0 = int(np.ceil(0.1 * cnb0 * 1.4))
+    # the budget holds only the target-sized level-1 pool; every tile tied at the threshold exceeds it
+    g = jpabi.Amr(n=n0, re=1600.0, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.1, frac2=0.1,
+                  maxgb=8 * cap_nref0 * PB * 1.0001 / 2 ** 30)
     with pytest.raises(jpabi.JPError) as e:
-        g.init(np.ones(cnb0, np.float32))  # every tile tied at the threshold
+        g.init(np.ones(cnb0, np.float32))
     assert e.value.code == jpabi.JP_ERR_CAPACITY
+
+
+def test_level1_pool_grows_on_tied_init_and_regrid():
+    """Threshold ties beyond the level-1 pool grow it (the reference would write out of bounds), at init
+    and during a regrid whose migration still reads the old pool; bit-exact with the oracle's rule."""
+    n0 = 32
+    cnb0, ng1 = (n0 // 8) ** 3, (n0 // 4) ** 3
+    rng = np.random.default_rng(5)
+    s0, s1 = rng.random(cnb0).astype(np.float32), rng.random(ng1).astype(np.float32)
+    for prec in (jpabi.PREC_FP32, jpabi.PREC_FP16):
+        g, o = pair(n=n0, re=1600.0, prec=prec, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.1,
+                    frac2=0.3)
+        g.init(s0, s1)
+        assert o.init(s0, s1) == 0
+        cap1 = g.counts()["cap1"]
+        copy_state(o, g)
+        g.step(2)
+        o.step(2)
+        g.regrid(np.ones(cnb0, np.float32), s1)  # every coarse tile tied
+        assert o.regrid(np.ones(cnb0, np.float32), s1) == 0
+        c = g.counts()
+        assert c["nf1"] == 8 * cnb0 > cap1 and c["cap1"] >= c["nf1"]
+        same_tables(g, o)
+        same_state(g, o)
+        g.step(2)
+        o.step(2)
+        same_state(g, o)
+    g, o = pair(n=24, re=1600.0, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.1, frac2=0.1)
+    g.init(np.ones(27, np.float32))
+    assert o.init(np.ones(27, np.float32)) == 0
+    same_tables(g, o)
 
 
 def test_amr_bench_and_graph_replay():

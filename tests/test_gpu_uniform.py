@@ -143,5 +143,5 @@ def test_large_lattice_invariants():
     for _ in range(4):
         g.step(50)
         E, ens, m = g.diag()
-        assert abs(m - m0) / m0 < 1e-6
+        assert abs(m - m0) / m0 < 1e-5  # FP32 round-off only (the reference's gate allows 5e-4)
     assert E < E0

@@ -141,11 +141,57 @@ def test_regrid_with_same_sensor_keeps_tables_and_copies_tiles():
     assert np.array_equal(a.raw(1), r1) and np.array_equal(a.raw(2), r2)
 
 
+PB = 19 * 4 * (2 * 512 + 1000) * 6 / 5  # pool bytes per fine tile as the budget counts them
+
+
 def test_overflow_is_reported():
     n0 = 24
     cnb0 = (n0 // 8) ** 3
+    cap_nref0 = int(np.ceil(0.1 * cnb0 * 1.4))
+    # a budget that holds only the target-sized level-1 pool: all tied marks every tile, beyond it
+    a = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.1, frac2=0.1,
+                budget_bytes=8 * cap_nref0 * PB * 1.0001)
+    assert a.counts()["cap1"] == 8 * cap_nref0
+    assert a.init(sen0=np.ones(cnb0, np.float32)) == 3
+
+
+def test_level1_pool_grows_on_tied_init():
+    n0 = 24
+    cnb0 = (n0 // 8) ** 3
     a = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.1, frac2=0.1)
-    assert a.init(sen0=np.ones(cnb0, np.float32)) == 3  # all tied: every tile marked, above cap_nref0
+    cap1 = a.counts()["cap1"]
+    assert a.init(sen0=np.ones(cnb0, np.float32)) == 0  # all tied: every tile marked
+    c = a.counts()
+    assert c["nf1"] == 8 * cnb0 > cap1 and c["cap1"] == 8 * cnb0 and c["cap_nref0"] == cap1 // 8
+    check_table_invariants(a, n0)
+    a.step(2)
+    assert np.all(np.isfinite(a.diag()))
+
+
+def test_level1_pool_grows_on_tied_regrid_and_keeps_the_data():
+    """Growth during a regrid: tiles that stay refined keep their level-1 populations (the migration
+    reads the old pool), new ones are prolonged from level 0."""
+    n0 = 32
+    cnb0, ng1 = (n0 // 8) ** 3, (n0 // 4) ** 3
+    rng = np.random.default_rng(5)
+    s0, s1 = rng.random(cnb0).astype(np.float32), rng.random(ng1).astype(np.float32)
+    a = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.1, frac2=0.3)
+    assert a.init(s0, s1) == 0
+    a.step(2)
+    cap1 = a.counts()["cap1"]
+    old_pos, old_f1 = a.table("f1pos"), a.raw(1)
+    assert a.regrid(np.ones(cnb0, np.float32), s1) == 0
+    c = a.counts()
+    assert c["nf1"] == 8 * cnb0 > cap1 and c["cap1"] >= c["nf1"]
+    check_table_invariants(a, n0)
+    new_pos, new_f1 = a.table("f1pos"), a.raw(1)
+    blk = 19 * 512
+    where = {int(p): b for b, p in enumerate(new_pos)}
+    for b, p in enumerate(old_pos):
+        nb = where[int(p)]
+        assert np.array_equal(new_f1[nb * blk:(nb + 1) * blk], old_f1[b * blk:(b + 1) * blk])
+    a.step(2)
+    assert np.all(np.isfinite(a.diag()))
 
 
 def test_level2_pool_grows_within_budget_and_overflows_beyond():
@@ -161,7 +207,6 @@ def test_level2_pool_grows_within_budget_and_overflows_beyond():
     assert c["nf2"] > cap2 and c["cap2"] >= c["nf2"]
     check_table_invariants(a, n0)
     # with a budget that only fits the initial pools, the same regrid is an overflow
-    PB = 19 * 4 * (2 * 512 + 1000) * 6 / 5
     b = orc.Amr(n0, re=1600, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.5, frac2=0.05,
                 budget_bytes=(a.counts()["cap1"] + cap2) * PB * 1.0001)
     assert b.init(s0, s1) == 0
