@@ -71,6 +71,7 @@ static int alloc_l2_pool(jp_ctx* c) {
   if (int rc = ctx_alloc_t(c, &a.l2oct, a.cap2, "l2oct")) return rc;
   if (int rc = ctx_alloc_t(c, &a.nb2, size_t(a.cap2) * kNbr, "nb2")) return rc;
   a.cap2_alloc = a.cap2;
+  ++a.pool_gen;
   return JP_OK;
 }
 static void free_l2_pool(jp_ctx* c) {
@@ -102,6 +103,7 @@ static int finish_tables(jp_ctx* c) {
 static int alloc_l1_pool(jp_ctx* c, bool with_f1a) {
   AmrState& a = c->amr;
   const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
+  ++a.pool_gen;
   if (with_f1a)
     if (int rc = ctx_alloc(c, &a.f1a, size_t(a.cap1) * tile * e, "f1a")) return rc;
   if (int rc = ctx_alloc(c, &a.f1b, size_t(a.cap1) * tile * e, "f1b")) return rc;
@@ -264,6 +266,7 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     ctx_free(c, a.f1b);
     if (int r = ctx_alloc(c, &a.f1b, size_t(a.cap1) * Q * MB3 * prec_bytes(c->prec), "f1b")) return r;
     a.f1_stale = false;
+    ++a.pool_gen;
   }
   a.nref1 = a.nf2 = 0;
   if (a.nf1) {
@@ -302,6 +305,7 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       HCK(c, hipStreamSynchronize(c->st));
       ctx_free(c, a.f2b);
       if (int r = ctx_alloc(c, &a.f2b, size_t(a.cap2) * Q * MB3 * prec_bytes(c->prec), "f2b")) return r;
+      ++a.pool_gen;
     }
   }
   if (int r = finish_tables(c)) return r;

@@ -22,7 +22,8 @@ struct AmrState {
   long hw_nf1 = 0, hw_nf2 = 0;
   int nref0 = 0, nf1 = 0, nref1 = 0, nf2 = 0, n_skin0 = 0, n_skin1 = 0;
   bool f1_stale = false;
-  int cap2_alloc = 0;  // level-2 tiles the allocated pool holds  // f1b still has a pre-growth size (until the regrid's migration completes)
+  int cap2_alloc = 0;  // level-2 tiles the allocated pool holds
+  int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)  // f1b still has a pre-growth size (until the regrid's migration completes)
   void *f1a = nullptr, *f1b = nullptr, *f2a = nullptr, *f2b = nullptr;  // stored populations
   void *prep0 = nullptr, *prep1 = nullptr, *gh1 = nullptr, *gh2 = nullptr;  // working-type values
   int *ref0 = nullptr, *fof0 = nullptr, *rl0 = nullptr, *l1pcb = nullptr, *l1oct = nullptr, *f1pos = nullptr;

@@ -86,7 +86,8 @@ static std::vector<long> signature(const EnsState& e) {
   std::vector<long> s;
   for (const jp_ctx* m : e.m) {
     const AmrState& a = m->amr;
-    s.insert(s.end(), {long(a.nf1), long(a.nf2), long(a.n_skin0), long(a.n_skin1), long(a.nref0), long(a.nref1)});
+    s.insert(s.end(), {long(a.nf1), long(a.nf2), long(a.n_skin0), long(a.n_skin1), long(a.nref0), long(a.nref1),
+                       long(a.pool_gen)});
   }
   return s;
 }
