@@ -55,15 +55,6 @@ __global__ __launch_bounds__(256) void k_fill_l2(const R* __restrict__ prep1, R*
                                                  const uint32_t* __restrict__ slots, AmrGeo G) {
   fill_l2_tile<R>(prep1, gh2, l2pl1, l2oct, f1pos, l1grid, ref1, slots, G, blockIdx.x, threadIdx.x, 256);
 }
-template <class S, bool kToDense>
-__global__ __launch_bounds__(256) void k_restrict(typename S::raw* __restrict__ cf, const typename S::raw* __restrict__ ff,
-                                                  const int* __restrict__ rl, const int* __restrict__ fof, int nref,
-                                                  AmrGeo G, float nqi, float cenc, float fdec) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= nref * MB3) return;
-  restrict_cell<S, kToDense>(cf, ff, rl, fof, G, nqi, cenc, fdec, t / MB3, t % MB3);
-}
-
 // prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
 // non-equilibrium regularised and rescaled by nqs.
 template <class S> __device__ inline void prolong_l0(const typename S::raw* __restrict__ c0, const AmrGeo& G, int gfi,
@@ -320,20 +311,6 @@ hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, 
   if (nf2 <= 0) return hipSuccess;
   JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l2<typename S::real>), dim3(nf2), dim3(256), 0, st, REAL(prep1), WREAL(gh2),
                                        l2pl1, l2oct, f1pos, l1grid, ref1, slots, G));
-  return hipGetLastError();
-}
-
-hipError_t amr_restrict(int prec, bool to_dense, const AmrGeo& G, void* cf, const void* ff, const int* rl, const int* fof,
-                        int nref, float nqi, float cenc, float fdec, hipStream_t st) {
-  if (nref <= 0) return hipSuccess;
-  const dim3 grid((nref * MB3 + 255) / 256);
-  if (to_dense) {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_restrict<S, true>), grid, dim3(256), 0, st, WRAW(cf), RAW(ff), rl, fof, nref, G,
-                                         nqi, cenc, fdec));
-  } else {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_restrict<S, false>), grid, dim3(256), 0, st, WRAW(cf), RAW(ff), rl, fof, nref, G,
-                                         nqi, cenc, fdec));
-  }
   return hipGetLastError();
 }
 

@@ -46,27 +46,26 @@ template <class S> __device__ inline void load_dense(const typename S::raw* __re
   });
 }
 
-// BGK collision of pulled populations, stored to tile b (src/tgv3d_amr.cu:268-273).
-// keep (optional, LDS [19][512]): receives each stored value as it reads back, for the fused restriction.
+// BGK collision of pulled values (the form's working representation), stored to tile b
+// (src/tgv3d_amr.cu:268-273). keep (optional, LDS [19][512]): receives each stored population as it reads
+// back, for the fused restriction.
 template <class S> __device__ inline void collide_store(typename S::real* fin, typename S::real om,
                                                         typename S::raw* __restrict__ fb, int b, int cell, float enc,
                                                         typename S::real* keep = nullptr, float dec = 1.f) {
-  using R = typename S::real;
-  const Mom<R> m = moments(fin);
-  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
-  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  const Collider<S> col(fin);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const typename S::raw v = S::template store<q>(fin[q] - om * (fin[q] - feq<q>(e)), enc);
+    const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], om), enc);
     fb[tidx(b, q, cell)] = v;
     if (keep) keep[q * MB3 + cell] = S::template load<q>(v, dec);
   });
 }
 
 // Restriction of one tile's 64 parent cells from the tile's just-stored populations held in LDS
-// (src/tgv3d_amr.cu:275-298, 358-381): the 8 children of a parent cell lie in this tile, so the average
-// needs no second pass over the fine level. The 512 lanes first sum (in the reference's child order)
-// 64 parents x 19 directions, then 64 lanes finish one parent cell each. Call with all lanes.
+// (src/tgv3d_amr.cu:275-298, 358-381): parent = feq + nqi * regularize(mean of the 8 children - feq).
+// The 8 children of a parent cell lie in this tile, so the average needs no second pass over the fine
+// level. The 512 lanes first sum (in the reference's child order) 64 parents x 19 directions, then 64
+// lanes finish one parent cell each. Call with all lanes.
 template <class S, bool kToDense>
 __device__ inline void restrict_tile_lds(typename S::real* lds, int cell, typename S::raw* __restrict__ cf, int pcb,
                                          int oct, const AmrGeo& G, float nqi, float cenc) {
@@ -128,15 +127,15 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
     constexpr int q = decltype(qc)::value;
     constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
     if constexpr (q == 0) {
-      fin[0] = S::template load<0>(fa[tidx(b, 0, cell)], dec);
+      fin[0] = S::template in<0>(fa[tidx(b, 0, cell)], dec);
     } else {
       const int si = i - cx, sj = j - cy, sk = k - cz;
       const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
       const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
       const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
       const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-      if (t >= 0) fin[q] = S::template load<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
-      else fin[q] = g[ghost_slot(q, i, j, k)];
+      if (t >= 0) fin[q] = S::template in<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
+      else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
     }
   });
   collide_store<S>(fin, om, fb, b, cell, enc, keep, dec);
@@ -242,45 +241,6 @@ __device__ inline void fill_l2_tile(const R* __restrict__ prep1, R* __restrict__
     if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
     gh2[size_t(b) * kGhostSlots + s] = out;
   }
-}
-
-// Restriction (src/tgv3d_amr.cu:275-298, 358-381): each covered parent cell = mean of its 8 children,
-// then feq + nqi * regularize(mean - feq). One lane per parent cell.
-template <class S, bool kToDense>
-__device__ inline void restrict_cell(typename S::raw* __restrict__ cf, const typename S::raw* __restrict__ ff,
-                                     const int* __restrict__ rl, const int* __restrict__ fof, const AmrGeo& G, float nqi,
-                                     float cenc, float fdec, int r, int cell) {
-  using R = typename S::real;
-  const int cb = rl[r];
-  const int lci = cell & 7, lcj = (cell >> 3) & 7, lck = cell >> 6;
-  const int oct = (lck / HALF) * 4 + (lcj / HALF) * 2 + (lci / HALF), fb = fof[cb] + oct;
-  const int fi0 = 2 * (lci % HALF), fj0 = 2 * (lcj % HALF), fk0 = 2 * (lck % HALF);
-  R avg[Q], nq[Q];
-  sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0); });
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
-    const int sc = ((fk0 + dk) * MB + (fj0 + dj)) * MB + (fi0 + di);
-    sfor<Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      avg[q] += R(0.125) * S::template load<q>(ff[tidx(fb, q, sc)], fdec);
-    });
-  }
-  const Mom<R> m = moments(avg);
-  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
-  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    nq[q] = avg[q] - feq<q>(e);
-  });
-  regularize(nq);
-  const uint32_t di = kToDense ? dense_of_tile(G, cb, cell) : 0;
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
-    if constexpr (kToDense) cf[uint64_t(q) * G.g0.plane + di] = v;
-    else cf[tidx(cb, q, cell)] = v;
-  });
 }
 
 }  // namespace jp

@@ -58,9 +58,6 @@ hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* prep0, void* gh1, 
 hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, const int* l2pl1, const int* l2oct,
                        const int* f1pos, const int* l1grid, const int* ref1, const uint32_t* slots, int nf2,
                        hipStream_t st);
-// level 1 -> 0 (coarse dense) when to_dense, else level 2 -> 1
-hipError_t amr_restrict(int prec, bool to_dense, const AmrGeo& G, void* cf, const void* ff, const int* rl,
-                        const int* fof, int nref, float nqi, float cenc, float fdec, hipStream_t st);
 hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct,
                        int nf1, float nqs, float dec0, float enc1, hipStream_t st);
 hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,

@@ -123,7 +123,7 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
   c->d.tau0 = 3.0 * c->d.nu + 0.5;
   c->d.omega0 = (p->prec == JP_PREC_FP64) ? 1.0 / c->d.tau0 : double(float(1.0 / c->d.tau0));
   c->d.kf = 2.0 * M_PI / double(N);
-  c->g0 = make_dense(N, 0);
+  c->d.cells = long(N) * N * N;
   c->variant = default_variant(c->prec);
   const double s = (p->prec == JP_PREC_FP16 || p->prec == JP_PREC_BF16) ? (p->scale > 0 ? p->scale : default_scale(p->u0)) : 1.0;
   if (s != std::exp2(std::round(std::log2(s)))) return fail(c, JP_ERR_ARG, "scale must be a power of two");

@@ -57,19 +57,19 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
     else w = *vp;
     R v[V];
 #pragma unroll
-    for (int j = 0; j < V; ++j) v[j] = S::template load<q>(w[j], dec);
+    for (int j = 0; j < V; ++j) v[j] = S::template in<q>(w[j], dec);
     if constexpr (cx == 0) {
 #pragma unroll
       for (int j = 0; j < V; ++j) f[q][j] = v[j];
     } else if constexpr (cx > 0) {  // source x - 1
       R l = __shfl_up(v[V - 1], 1, 64);
-      if (left_edge) l = S::template load<q>(base[xl], dec);
+      if (left_edge) l = S::template in<q>(base[xl], dec);
       f[q][0] = l;
 #pragma unroll
       for (int j = 1; j < V; ++j) f[q][j] = v[j - 1];
     } else {  // source x + 1
       R r = __shfl_down(v[0], 1, 64);
-      if (right_edge) r = S::template load<q>(base[xr], dec);
+      if (right_edge) r = S::template in<q>(base[xr], dec);
 #pragma unroll
       for (int j = 0; j < V - 1; ++j) f[q][j] = v[j + 1];
       f[q][V - 1] = r;
@@ -79,19 +79,17 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
   for (int j = 0; j < V; ++j) {
     R fin[Q];
     sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = f[decltype(qc)::value][j]; });
-    const Mom<R> m = moments(fin);
-    const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
-    const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+    const Collider<S> col(fin);
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      f[q][j] = fin[q] - omega * (fin[q] - feq<q>(e));
+      f[q][j] = col.template post<q>(fin[q], omega);
     });
   }
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     vec w;
 #pragma unroll
-    for (int j = 0; j < V; ++j) w[j] = S::template store<q>(f[q][j], enc);
+    for (int j = 0; j < V; ++j) w[j] = S::template out<q>(f[q][j], enc);
     vec* p = reinterpret_cast<vec*>(fb + uint64_t(q) * g.plane + cell0);
     if constexpr (kNtStore) __builtin_nontemporal_store(w, p);
     else *p = w;
@@ -111,18 +109,16 @@ __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, t
     constexpr int q = decltype(qc)::value;
     const uint32_t src = n.zs[CZ(q) + 1] + n.ys[CY(q) + 1] + n.xs[CX(q) + 1];
     const typename S::raw* p = fa + uint64_t(q) * g.plane + src;
-    if constexpr (kNtLoad) fin[q] = S::template load<q>(__builtin_nontemporal_load(p), dec);
-    else fin[q] = S::template load<q>(*p, dec);
+    if constexpr (kNtLoad) fin[q] = S::template in<q>(__builtin_nontemporal_load(p), dec);
+    else fin[q] = S::template in<q>(*p, dec);
   });
-  const Mom<R> m = moments(fin);
-  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
-  const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
+  const Collider<S> col(fin);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const R fo = fin[q] - omega * (fin[q] - feq<q>(e));
+    const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], omega), enc);
     typename S::raw* p = fb + uint64_t(q) * g.plane + cell;
-    if constexpr (kNtStore) __builtin_nontemporal_store(S::template store<q>(fo, enc), p);
-    else *p = S::template store<q>(fo, enc);
+    if constexpr (kNtStore) __builtin_nontemporal_store(v, p);
+    else *p = v;
   });
 }
 

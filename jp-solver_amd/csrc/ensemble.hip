@@ -76,18 +76,30 @@ __global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict
                         d.dec[2], w.y, threadIdx.x);
   }
 }
-// items: (member, 2 * refined tile + half): 256 parent cells per item
+// Second substep of a fine level with the restriction into its parent fused in (the single-context
+// k_fine_step_r): items (member, fine tile); pf: the fine level's parity, pc: the parent level's.
 template <class S, int kLevel>
-__global__ __launch_bounds__(256) void kb_restrict(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
-                                                   int pc, int pf) {
+__global__ __launch_bounds__(512) void kb_fine_step_r(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+                                                      int n, int pf, int pc) {
+  using R = typename S::real;
+  __shared__ int snb[kNbr];
+  __shared__ R keep[Q * MB3];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
-    const int r = w.y >> 1, cell = (w.y & 1) * 256 + threadIdx.x;
-    if constexpr (kLevel == 1)  // level 1 -> dense level 0
-      restrict_cell<S, true>(wr<S>(d.c0[pc]), cr<S>(d.f1[pf]), d.rl0, d.fof0, d.G, d.nqi10, d.enc[0], d.dec[1], r, cell);
-    else
-      restrict_cell<S, false>(wr<S>(d.f1[pc]), cr<S>(d.f2[pf]), d.rl1, d.fof1, d.G, d.nqi21, d.enc[1], d.dec[2], r, cell);
+    const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
+    __syncthreads();  // the previous item's lanes are done with snb and keep
+    if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(w.y) * kNbr + threadIdx.x];
+    __syncthreads();
+    if constexpr (kLevel == 1) {
+      fine_step_cell<S>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]), d.enc[1],
+                        d.dec[1], w.y, threadIdx.x, keep);
+      restrict_tile_lds<S, true>(keep, threadIdx.x, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
+    } else {
+      fine_step_cell<S>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]), d.enc[2],
+                        d.dec[2], w.y, threadIdx.x, keep);
+      restrict_tile_lds<S, false>(keep, threadIdx.x, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
+    }
   }
 }
 
@@ -118,11 +130,11 @@ hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* i
       break;
     case EK_STEP1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 1>), g, dim3(512), 0, st, md, items, n, pa)); break;
     case EK_STEP2: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 2>), g, dim3(512), 0, st, md, items, n, pa)); break;
-    case EK_RESTR1:
-      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_restrict<S, 1>), g, dim3(256), 0, st, md, items, n, pa, pb));
+    case EK_STEP1R:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step_r<S, 1>), g, dim3(512), 0, st, md, items, n, pa, pb));
       break;
-    case EK_RESTR2:
-      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_restrict<S, 2>), g, dim3(256), 0, st, md, items, n, pa, pb));
+    case EK_STEP2R:
+      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step_r<S, 2>), g, dim3(512), 0, st, md, items, n, pa, pb));
       break;
     default: return hipErrorInvalidValue;
   }

@@ -21,10 +21,11 @@ struct MemberDesc {
   float nqs01, nqi10, nqs12, nqi21;
 };
 
-enum EnsKernel { EK_DENSE = 0, EK_PREP0, EK_FILL1, EK_PREP1, EK_FILL2, EK_STEP2, EK_RESTR2, EK_STEP1, EK_RESTR1, EK_COUNT };
+// EK_STEP2R / EK_STEP1R: the second substep of a fine level with the restriction into its parent fused in.
+enum EnsKernel { EK_DENSE = 0, EK_PREP0, EK_FILL1, EK_PREP1, EK_FILL2, EK_STEP2, EK_STEP2R, EK_STEP1, EK_STEP1R, EK_COUNT };
 
 // One batched launch of kernel k over n items with at most `groups` workgroups. pa: parity of the
-// buffer read (for restriction: the parent level's), pb: the fine level's parity (restriction only).
+// buffer read, pb: the parent level's parity (the fused restriction's target; EK_STEP2R / EK_STEP1R only).
 hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* items, int n, int groups, int pa, int pb,
                       hipStream_t st);
 

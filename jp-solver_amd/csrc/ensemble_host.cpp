@@ -108,13 +108,13 @@ static int upload(jp_ctx* c) {
       for (int k = 0; k < a.n_skin0; ++k) L[EK_PREP0].push_back(make_int2(i, k));
       for (int b = 0; b < a.nf1; ++b) L[EK_FILL1].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf1; ++b) L[EK_STEP1].push_back(make_int2(i, b));
-      for (int r = 0; r < 2 * a.nref0; ++r) L[EK_RESTR1].push_back(make_int2(i, r));
+      for (int b = 0; b < a.nf1; ++b) L[EK_STEP1R].push_back(make_int2(i, b));
     }
     if (a.nf1 && a.nf2) {
       for (int k = 0; k < a.n_skin1; ++k) L[EK_PREP1].push_back(make_int2(i, k));
       for (int b = 0; b < a.nf2; ++b) L[EK_FILL2].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf2; ++b) L[EK_STEP2].push_back(make_int2(i, b));
-      for (int r = 0; r < 2 * a.nref1; ++r) L[EK_RESTR2].push_back(make_int2(i, r));
+      for (int b = 0; b < a.nf2; ++b) L[EK_STEP2R].push_back(make_int2(i, b));
     }
   }
   HCK(c, hipStreamSynchronize(c->st));
@@ -174,15 +174,13 @@ int ens_step_once(jp_ctx* c) {
   for (int s1 = 0; s1 < 2; ++s1) {
     HCK(c, L(EK_PREP1, e.p1, 0));
     HCK(c, L(EK_FILL2, 0, 0));
-    for (int s2 = 0; s2 < 2; ++s2) {
-      HCK(c, L(EK_STEP2, e.p2, 0));
-      swap_all(e, 2);
-    }
-    HCK(c, L(EK_RESTR2, e.p1, e.p2));
-    HCK(c, L(EK_STEP1, e.p1, 0));
+    HCK(c, L(EK_STEP2, e.p2, 0));
+    swap_all(e, 2);
+    HCK(c, L(EK_STEP2R, e.p2, e.p1));  // second level-2 substep, restricted into level 1
+    swap_all(e, 2);
+    HCK(c, L(s1 ? EK_STEP1R : EK_STEP1, e.p1, e.p0));  // the second one restricts into level 0
     swap_all(e, 1);
   }
-  HCK(c, L(EK_RESTR1, e.p0, e.p1));
   ++e.steps;
   return JP_OK;
 }
@@ -237,13 +235,13 @@ int ens_autotune(jp_ctx* c, int* out) {
   HCK(c, hipEventCreate(&a));
   HCK(c, hipEventCreate(&b));
   for (int k = 0; k < EK_COUNT; ++k) {
-    if (k == EK_RESTR2) continue;
+    if (k == EK_STEP2R || k == EK_STEP1R) continue;  // they write the parent level: tuned as their unfused twins
     if (e.n[k] == 0) {
       e.groups[k] = cu * 4;
       continue;
     }
-    const int pa = (k == EK_DENSE) ? e.p0 : (k == EK_PREP0 || k == EK_RESTR1) ? e.p0 : (k == EK_STEP2) ? e.p2 : e.p1;
-    const int pb = (k == EK_RESTR1) ? e.p1 : 0;
+    const int pa = (k == EK_DENSE || k == EK_PREP0) ? e.p0 : (k == EK_STEP2) ? e.p2 : e.p1;
+    const int pb = 0;
     float best = 1e30f;
     int bestg = cu * 4;
     for (int g : cand) {
@@ -265,7 +263,8 @@ int ens_autotune(jp_ctx* c, int* out) {
     }
     e.groups[k] = bestg;
   }
-  e.groups[EK_RESTR2] = e.groups[EK_RESTR1];
+  e.groups[EK_STEP2R] = e.groups[EK_STEP2];
+  e.groups[EK_STEP1R] = e.groups[EK_STEP1];
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   drop_graph(c);

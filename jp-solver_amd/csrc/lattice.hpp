@@ -116,6 +116,16 @@ template <int q, class R> __device__ inline R feq(const EqPre<R>& e) {
   }
 }
 
+// Equilibrium minus the weight, w_q ((rho - 1) + rho (cu + cu^2/2 - 1.5 u^2)), for the shifted forms.
+template <int q, class R> __device__ inline R deq(const EqPre<R>& e, R drho, R rho) {
+  if constexpr (q == 0) {
+    return W<R>(0) * (drho + rho * (R(0) - e.t15));
+  } else {
+    R cu = R(3) * cdot<q>(e.ux, e.uy, e.uz);
+    return W<R>(q) * (drho + rho * (cu + R(0.5) * cu * cu - e.t15));
+  }
+}
+
 // Second-order Hermite projection of a non-equilibrium vector (src/tgv3d_amr.cu:56-67).
 template <class R> __device__ inline void regularize(R* n) {
   R Pxx = R(0), Pyy = R(0), Pzz = R(0), Pxy = R(0), Pxz = R(0), Pyz = R(0);
@@ -139,30 +149,45 @@ template <class R> __device__ inline void regularize(R* n) {
 }
 
 // ---- storage forms ----
-// A form maps the stored element of direction q to the working type and back. FP16/BF16 hold the
-// shifted value (f - w_q) * enc with enc a power of two; dec = 1/enc exactly.
+// A form maps the stored element of direction q to the working type and back: load / store convert
+// populations f. FP16/BF16 hold the shifted value (f - w_q) * enc with enc a power of two (dec = 1/enc
+// exactly), and collide in that shifted representation (kShift): in / out convert the collision's
+// working value (f for the plain forms, d = f - w_q for the shifted ones), from_full maps a population
+// f (ghost values) to it.
 struct FormF32 {
   using raw = float;
   using real = float;
   static constexpr int id = 0;
+  static constexpr bool kShift = false;
   template <int q> __device__ static inline real load(raw r, float) { return r; }
   template <int q> __device__ static inline raw store(real f, float) { return f; }
+  template <int q> __device__ static inline real in(raw r, float) { return r; }
+  template <int q> __device__ static inline raw out(real f, float) { return f; }
+  template <int q> __device__ static inline real from_full(real f) { return f; }
 };
 struct FormF64 {
   using raw = double;
   using real = double;
   static constexpr int id = 3;
+  static constexpr bool kShift = false;
   template <int q> __device__ static inline real load(raw r, float) { return r; }
   template <int q> __device__ static inline raw store(real f, float) { return f; }
+  template <int q> __device__ static inline real in(raw r, float) { return r; }
+  template <int q> __device__ static inline raw out(real f, float) { return f; }
+  template <int q> __device__ static inline real from_full(real f) { return f; }
 };
 struct FormF16 {
   using raw = _Float16;
   using real = float;
   static constexpr int id = 1;
+  static constexpr bool kShift = true;
   template <int q> __device__ static inline real load(raw r, float dec) { return float(r) * dec + W<float>(q); }
   template <int q> __device__ static inline raw store(real f, float enc) {
     return static_cast<_Float16>((f - W<float>(q)) * enc);
   }
+  template <int q> __device__ static inline real in(raw r, float dec) { return float(r) * dec; }
+  template <int q> __device__ static inline raw out(real d, float enc) { return static_cast<_Float16>(d * enc); }
+  template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
 };
 __device__ inline uint16_t bf16_rne(float x) {
   uint32_t u = __float_as_uint(x);
@@ -174,10 +199,41 @@ struct FormBF16 {
   using raw = uint16_t;
   using real = float;
   static constexpr int id = 2;
+  static constexpr bool kShift = true;
   template <int q> __device__ static inline real load(raw r, float dec) {
     return __uint_as_float(uint32_t(r) << 16) * dec + W<float>(q);
   }
   template <int q> __device__ static inline raw store(real f, float enc) { return bf16_rne((f - W<float>(q)) * enc); }
+  template <int q> __device__ static inline real in(raw r, float dec) { return __uint_as_float(uint32_t(r) << 16) * dec; }
+  template <int q> __device__ static inline raw out(real d, float enc) { return bf16_rne(d * enc); }
+  template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
+};
+
+// BGK collision of one cell in the form's working representation (src/tgv3d_uniform.cu:55-60,
+// src/tgv3d_amr.cu:268-273): post<q>(x) = x - omega (x - eq_q). The shifted forms collide the
+// deviations d_q = f_q - w_q: rho - 1 = sum d_q, rho u = sum c_q d_q, eq_q = feq_q - w_q. In exact
+// arithmetic this is the same collision; in FP32 every operand is then of the order of the deviations,
+// whereas summing w_q + d_q rounds rho at the fixed low bits of the w_q, a bias of ~1e-7 per cell and
+// step that would show as a linear drift of the mass.
+template <class S> struct Collider {
+  using R = typename S::real;
+  EqPre<R> e;
+  R drho, rho;
+  __device__ explicit Collider(const R* fin) {
+    const Mom<R> m = moments(fin);
+    if constexpr (S::kShift) {
+      drho = m.rho;
+      rho = R(1) + m.rho;
+    } else {
+      drho = R(0);
+      rho = m.rho;
+    }
+    e = eq_pre<R>(rho, m.ux / rho, m.uy / rho, m.uz / rho);
+  }
+  template <int q> __device__ R post(R x, R om) const {
+    if constexpr (S::kShift) return x - om * (x - deq<q>(e, drho, rho));
+    else return x - om * (x - feq<q>(e));
+  }
 };
 
 // Runtime-q decode for kernels that gather one direction at a time.

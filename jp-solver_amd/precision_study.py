@@ -15,6 +15,8 @@ import sys
 import time
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import numpy as np  # noqa: E402
+
 import jpabi  # noqa: E402
 
 FORMS = ["fp64", "fp32", "fp16", "bf16"]
@@ -41,17 +43,40 @@ def run_uniform(n, re, u0, tstar, every, prec):
     return {"steps": steps, "curve": curve, "wall_s": wall}
 
 
-def run_amr(n0, re, u0, tstar, every, prec, adapt):
+def sensors_by_pos(a, n0):
+    """The sensors the last init/regrid used: level 0 per coarse tile, level 1 scattered by position."""
+    ng1 = (n0 // 4) ** 3
+    s1 = np.zeros(ng1, np.float32)
+    pos = a.table("f1pos")
+    if pos.size:
+        s1[pos] = a.table("sen1")
+    return a.table("sen0"), s1
+
+
+def run_amr(n0, re, u0, tstar, every, prec, adapt, script=None):
+    """Dynamic sensor AMR. The FP64 run records the sensor fields of its init and of every regrid
+    (script=None); the other forms replay them, so all forms advance on the same sequence of grids
+    and the comparison measures storage error, not a different refinement history (threshold ties
+    of the symmetric TGV are otherwise broken differently by each form's rounding)."""
     a = jpabi.Amr(n0, re=re, u0=u0, l1=jpabi.MODE_SENSOR, l2=jpabi.MODE_SENSOR, frac1=0.3, frac2=0.1, adapt=adapt,
                   prec=jpabi.PREC_NAMES[prec])
-    a.init()
+    rec = []
+    if script is None:
+        a.init()
+        rec.append(sensors_by_pos(a, n0))
+    else:
+        a.init(*script[0])
     d = a.derived()
     steps = int(tstar / (u0 * d["kf"]) + 0.5)
     curve = []
     t0 = time.time()
     for s in range(steps + 1):
         if s > 0 and s % adapt == 0:
-            a.regrid()
+            if script is None:
+                a.regrid()
+                rec.append(sensors_by_pos(a, n0))
+            else:
+                a.regrid(*script[s // adapt])  # entry 0 is the init's
         if s % every == 0:
             E, ens, m = a.diag()
             curve.append((s * u0 * d["kf"], E, ens, m))
@@ -60,7 +85,7 @@ def run_amr(n0, re, u0, tstar, every, prec, adapt):
     wall = time.time() - t0
     c = a.counts()
     a.close()
-    return {"steps": steps, "curve": curve, "wall_s": wall, "counts": c}
+    return {"steps": steps, "curve": curve, "wall_s": wall, "counts": c, "script": rec}
 
 
 def summarise(runs):
@@ -106,7 +131,8 @@ def main():
         aruns = {}
         for f in FORMS:
             print(f"# amr N0={args.amr} {f}", flush=True)
-            aruns[f] = run_amr(args.amr, args.re, args.u0, args.tstar, args.every, f, args.adapt)
+            aruns[f] = run_amr(args.amr, args.re, args.u0, args.tstar, args.every, f, args.adapt,
+                               None if f == "fp64" else aruns["fp64"]["script"])
         res["amr"] = {"summary": summarise(aruns), "curves": {f: r["curve"] for f, r in aruns.items()},
                       "counts": {f: r["counts"] for f, r in aruns.items()}}
     with open(args.out, "w") as fh:

@@ -316,10 +316,10 @@ template <class S> struct Amr {
           if (sref[sidx]) {
             int sqx = (sfi % (2 * MB)) / MB, sqy = (sfj % (2 * MB)) / MB, sqz = (sfk % (2 * MB)) / MB;
             int sfb = sfof[sidx] + (sqz * 4 + sqy * 2 + sqx), sc2 = ((sfk % MB) * MB + (sfj % MB)) * MB + (sfi % MB);
-            fin[q] = S::load(f1a[(size_t(sfb) * Q + q) * MB3 + sc2], q, sc.dec);
+            fin[q] = S::in(f1a[(size_t(sfb) * Q + q) * MB3 + sc2], q, sc.dec);
           } else {
             int ei = i - cvx[q] + 1, ej = j - cvy[q] + 1, ek = k - cvz[q] + 1;
-            fin[q] = gh1[(size_t(b) * Q + q) * EB3 + (ek * EB + ej) * EB + ei];
+            fin[q] = S::from_full(gh1[(size_t(b) * Q + q) * EB3 + (ek * EB + ej) * EB + ei], q);
           }
         }
         collide_store(fin, om1, f1b, b, cell, sc);
@@ -344,10 +344,10 @@ template <class S> struct Amr {
           if (lb >= 0 && ref1[lb]) {
             int sqx = (sfi % (2 * MB)) / MB, sqy = (sfj % (2 * MB)) / MB, sqz = (sfk % (2 * MB)) / MB;
             int sfb = fof1[lb] + (sqz * 4 + sqy * 2 + sqx), sc2 = ((sfk % MB) * MB + (sfj % MB)) * MB + (sfi % MB);
-            fin[q] = S::load(f2a[(size_t(sfb) * Q + q) * MB3 + sc2], q, sc.dec);
+            fin[q] = S::in(f2a[(size_t(sfb) * Q + q) * MB3 + sc2], q, sc.dec);
           } else {
             int ei = i - cvx[q] + 1, ej = j - cvy[q] + 1, ek = k - cvz[q] + 1;
-            fin[q] = gh2[(size_t(b) * Q + q) * EB3 + (ek * EB + ej) * EB + ei];
+            fin[q] = S::from_full(gh2[(size_t(b) * Q + q) * EB3 + (ek * EB + ej) * EB + ei], q);
           }
         }
         collide_store(fin, om2, f2b, b, cell, sc);
@@ -355,14 +355,10 @@ template <class S> struct Amr {
     }
     std::swap(f2a, f2b);
   }
+  // fin: pulled values in the form's working representation (src/tgv3d_amr.cu:268-273)
   static void collide_store(R* fin, R om, std::vector<raw>& out, int b, int cell, Scale sc) {
-    R rho, ux, uy, uz;
-    moments(fin, rho, ux, uy, uz);
-    ux /= rho;
-    uy /= rho;
-    uz /= rho;
-    for (int q = 0; q < Q; ++q)
-      out[(size_t(b) * Q + q) * MB3 + cell] = S::store(fin[q] - om * (fin[q] - feq_q<R>(q, rho, ux, uy, uz)), q, sc.enc);
+    collide<S>(fin, om);
+    for (int q = 0; q < Q; ++q) out[(size_t(b) * Q + q) * MB3 + cell] = S::out(fin[q], q, sc.enc);
   }
   // Restriction fine -> parent, src/tgv3d_amr.cu:275-298 (L1 -> L0) and 358-381 (L2 -> L1).
   void restrict_level(std::vector<raw>& cf, const std::vector<raw>& ff, const std::vector<int>& rl,

@@ -91,31 +91,50 @@ inline float bf16_to_f32(uint16_t h) {
   return f;
 }
 
-// A storage form: raw element type, compute type, decode / encode of one population of direction q.
+// A storage form: raw element type, compute type, decode / encode of one population of direction q
+// (load / store). The collision works in the form's own representation (in / out; from_full maps a
+// population, e.g. a ghost value, to it): populations f for the plain forms, deviations d = f - w_q for
+// the shifted FP16/BF16 forms (shift = true).
 struct SF32 {
   using raw = float;
   using real = float;
+  static constexpr bool shift = false;
   static float load(raw r, int, float) { return r; }
   static raw store(float f, int, float) { return f; }
+  static float in(raw r, int, float) { return r; }
+  static raw out(float f, int, float) { return f; }
+  static float from_full(float f, int) { return f; }
 };
 struct SF64 {
   using raw = double;
   using real = double;
+  static constexpr bool shift = false;
   static double load(raw r, int, float) { return r; }
   static raw store(double f, int, float) { return f; }
+  static double in(raw r, int, float) { return r; }
+  static raw out(double f, int, float) { return f; }
+  static double from_full(double f, int) { return f; }
 };
 // FP16 of (f - w_q) * scale; scale is a power of two, inv its exact inverse.
 struct SF16 {
   using raw = uint16_t;
   using real = float;
+  static constexpr bool shift = true;
   static float load(raw r, int q, float inv) { return f16_to_f32(r) * inv + wq<float>(q); }
   static raw store(float f, int q, float scale) { return f32_to_f16((f - wq<float>(q)) * scale); }
+  static float in(raw r, int, float inv) { return f16_to_f32(r) * inv; }
+  static raw out(float d, int, float scale) { return f32_to_f16(d * scale); }
+  static float from_full(float f, int q) { return f - wq<float>(q); }
 };
 struct SBF16 {
   using raw = uint16_t;
   using real = float;
+  static constexpr bool shift = true;
   static float load(raw r, int q, float inv) { return bf16_to_f32(r) * inv + wq<float>(q); }
   static raw store(float f, int q, float scale) { return f32_to_bf16((f - wq<float>(q)) * scale); }
+  static float in(raw r, int, float inv) { return bf16_to_f32(r) * inv; }
+  static raw out(float d, int, float scale) { return f32_to_bf16(d * scale); }
+  static float from_full(float f, int q) { return f - wq<float>(q); }
 };
 
 // Scaling of a stored level: enc multiplies before rounding, dec multiplies after widening.
@@ -157,6 +176,34 @@ template <class R> inline void moments(const R* f, R& rho, R& ux, R& uy, R& uz) 
     ux += R(cvx[q]) * f[q];
     uy += R(cvy[q]) * f[q];
     uz += R(cvz[q]) * f[q];
+  }
+}
+// feq_q - w_q = w_q ((rho - 1) + rho (cu + cu^2/2 - 1.5 u^2)), the equilibrium of the shifted forms.
+template <class R> inline R deq_q(int q, R drho, R rho, R ux, R uy, R uz) {
+  R cu = R(3) * (R(cvx[q]) * ux + R(cvy[q]) * uy + R(cvz[q]) * uz);
+  return wq<R>(q) * (drho + rho * (cu + R(0.5) * cu * cu - R(1.5) * (ux * ux + uy * uy + uz * uz)));
+}
+// BGK collision of one cell in the form's working representation, in place
+// (src/tgv3d_uniform.cu:55-60; src/tgv3d_amr.cu:268-273). The shifted forms collide the deviations
+// d_q = f_q - w_q (rho - 1 = sum d_q, rho u = sum c_q d_q): the same collision in exact arithmetic, but
+// in FP32 summing w_q + d_q would round rho at the fixed low bits of the w_q, a per-step bias of ~1e-7
+// that drifts the mass linearly.
+template <class S> inline void collide(typename S::real* fin, typename S::real om) {
+  using R = typename S::real;
+  R rho, ux, uy, uz;
+  moments(fin, rho, ux, uy, uz);
+  if (S::shift) {
+    const R drho = rho;
+    rho = R(1) + drho;
+    ux /= rho;
+    uy /= rho;
+    uz /= rho;
+    for (int q = 0; q < Q; ++q) fin[q] = fin[q] - om * (fin[q] - deq_q<R>(q, drho, rho, ux, uy, uz));
+  } else {
+    ux /= rho;
+    uy /= rho;
+    uz /= rho;
+    for (int q = 0; q < Q; ++q) fin[q] = fin[q] - om * (fin[q] - feq_q<R>(q, rho, ux, uy, uz));
   }
 }
 inline int pmod(int a, int n) { return ((a % n) + n) % n; }
@@ -208,16 +255,10 @@ void step_uniform(const typename S::raw* fa, typename S::raw* fb, int N, typenam
         int sbx = sgi / MB, sby = sgj / MB, sbz = sgk / MB;
         int nbk = (sbz * gx + sby) * gx + sbx;
         int scl = ((sgk - sbz * MB) * MB + (sgj - sby * MB)) * MB + (sgi - sbx * MB);
-        fin[q] = S::load(fa[(size_t(nbk) * Q + q) * MB3 + scl], q, sc.dec);
+        fin[q] = S::in(fa[(size_t(nbk) * Q + q) * MB3 + scl], q, sc.dec);
       }
-      R rho, ux, uy, uz;
-      moments(fin, rho, ux, uy, uz);
-      ux /= rho;
-      uy /= rho;
-      uz /= rho;
-      for (int q = 0; q < Q; ++q)
-        fb[(size_t(b) * Q + q) * MB3 + cell] =
-            S::store(fin[q] - omega * (fin[q] - feq_q<R>(q, rho, ux, uy, uz)), q, sc.enc);
+      collide<S>(fin, omega);
+      for (int q = 0; q < Q; ++q) fb[(size_t(b) * Q + q) * MB3 + cell] = S::out(fin[q], q, sc.enc);
     }
   }
 }

@@ -115,3 +115,58 @@ def test_fp32_and_fp64_agree_over_short_run():
     b.step(20)
     assert np.allclose(a.f64(), b.f64(), rtol=0, atol=2e-6)
     assert a.diag() == pytest.approx(b.diag(), rel=1e-4)
+
+
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def test_fp32_mass_drift_is_the_weight_sum_bias():
+    """The FP32 step keeps the reference's arithmetic, whose only systematic mass change is
+    omega (sum of the FP32 weights - 1) per cell and step."""
+    n = 16
+    a = orc.Uniform(n)
+    a.init()
+    m0 = a.diag()[2]
+    a.step(200)
+    drift = (a.diag()[2] - m0) / m0 / 200
+    om = np.float32(1.0 / (3 * 0.05 * n / (2 * math.pi * 1600) + 0.5))  # src/tgv3d_uniform.cu:151-152
+    bias = float(om) * (float(np.sum(W.astype(np.float32).astype(np.float64))) - 1.0)
+    assert drift == pytest.approx(bias, rel=0.05)
+
+
+@pytest.mark.parametrize("prec", [orc.PREC_FP16, orc.PREC_BF16])
+def test_shifted_forms_hold_mass(prec):
+    """The shifted forms collide the deviations f - w_q, so their mass changes only by unbiased rounding
+    (FP16: ~1e-7 over 400 steps at 32^3, against 1.2e-5 for the FP32 form's bias and 1.7e-4 for an FP32
+    collision of w_q + d_q on the stored values). BF16's 8-bit rounding of the slowly varying
+    deviations stays correlated over many steps (~5e-8 per step, measured), hence its looser bound."""
+    n = 32
+    a = orc.Uniform(n, prec=prec)
+    a.init()
+    m0 = a.diag()[2]
+    a.step(400)
+    assert abs(a.diag()[2] - m0) / m0 < (1e-6 if prec == orc.PREC_FP16 else 6e-5)
+
+
+@pytest.mark.parametrize("prec", [orc.PREC_FP16, orc.PREC_BF16])
+def test_shifted_step_is_the_fp64_step_within_storage_rounding(prec):
+    """One shifted-form step from a stored state equals the FP64 step of the decoded state, up to the
+    rounding of the stored deviation (half a unit in its last place) plus FP32 collision error."""
+    n = 16
+    h = orc.Uniform(n, prec=prec)
+    h.init()
+    h.step(10)
+    d = orc.Uniform(n, prec=orc.PREC_FP64)
+    d.set_raw(h.f64())
+    h.step(1)
+    d.step(1)
+    exact, got = d.f64(), h.f64()
+    scale = orc.default_scale(0.05)
+    w = np.repeat(W, 512)[None, :].repeat(n ** 3 // 512, axis=0).reshape(-1)
+    dev = (exact - w) * scale
+    if prec == orc.PREC_FP16:
+        half = np.spacing(np.abs(dev).astype(np.float16)).astype(np.float64) / 2
+    else:
+        e = np.floor(np.log2(np.maximum(np.abs(dev), 1e-30)))
+        half = 2.0 ** (e - 8)
+    assert np.all(np.abs(got - exact) <= half / scale * 1.0001 + 3e-8)
