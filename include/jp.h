@@ -51,6 +51,8 @@ typedef struct {
   double phase[3]; /* TGV phase shift per axis in radians (0 = the reference's initial state) */
   double scale;    /* FP16/BF16 store scale; 0 = default 2^floor(log2(2048/u0)) */
   int device;      /* HIP device ordinal */
+  long plane_pad;  /* elements appended to each of the 19 population planes, a multiple of 64 (HBM
+                      channel spread); 0 = the library's default for the lattice, -1 = none */
 } jp_uniform_params;
 
 /* Derived run constants: src/tgv3d_uniform.cu:151-153 (uniform) and src/tgv3d_amr.cu:659-664 (AMR). */
@@ -163,7 +165,10 @@ int jp_ensemble_create(const jp_member* members, int count, int prec, double max
 /* Borrowed handle of member `index` (owned by the ensemble, valid until jp_destroy of the ensemble). */
 int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
 /* Picks the workgroup count of every batched launch by timing candidates on the current state (each
- * timed kernel is re-run on unchanged inputs, so the state is unchanged); result in groups[9]. */
+ * timed kernel is re-run on unchanged inputs, so the state is unchanged); result in groups[JP_ENS_KERNELS]
+ * (dense step, level-1 fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step,
+ * level-1 step + restriction). */
+#define JP_ENS_KERNELS 7
 int jp_autotune(jp_ctx* ens, int* groups);
 
 int jp_get_counts(jp_ctx* c, jp_counts* out);

@@ -37,23 +37,23 @@ __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __re
   restrict_tile_lds<S, kToDense>(keep, threadIdx.x, cf, ptile[b], poct[b], G, nqi, cenc);
 }
 
-template <class S, bool kDense>
-__global__ __launch_bounds__(512) void k_prep(const typename S::raw* __restrict__ f, typename S::real* __restrict__ pr,
-                                              const int* __restrict__ list, AmrGeo G, float nqs, float dec) {
-  prep_cell<S, kDense>(f, pr, G, nqs, dec, list[blockIdx.x], threadIdx.x);
+// Ghost fills, one fine tile per workgroup (the parent box and prep values staged in LDS).
+template <class S>
+__global__ __launch_bounds__(256) void k_fill_l1(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
+                                                 const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
+                                                 const int* __restrict__ nb1, const uint32_t* __restrict__ slots, AmrGeo G,
+                                                 float nqs, float dec) {
+  __shared__ FillLds<typename S::real> L;
+  fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, blockIdx.x, threadIdx.x, 256, L);
 }
-template <class R>
-__global__ __launch_bounds__(256) void k_fill_l1(const R* __restrict__ prep0, R* __restrict__ gh1, const int* __restrict__ l1pcb,
-                                                 const int* __restrict__ l1oct, const int* __restrict__ ref0,
-                                                 const uint32_t* __restrict__ slots, AmrGeo G) {
-  fill_l1_tile<R>(prep0, gh1, l1pcb, l1oct, ref0, slots, G, blockIdx.x, threadIdx.x, 256);
-}
-template <class R>
-__global__ __launch_bounds__(256) void k_fill_l2(const R* __restrict__ prep1, R* __restrict__ gh2, const int* __restrict__ l2pl1,
-                                                 const int* __restrict__ l2oct, const int* __restrict__ f1pos,
-                                                 const int* __restrict__ l1grid, const int* __restrict__ ref1,
-                                                 const uint32_t* __restrict__ slots, AmrGeo G) {
-  fill_l2_tile<R>(prep1, gh2, l2pl1, l2oct, f1pos, l1grid, ref1, slots, G, blockIdx.x, threadIdx.x, 256);
+template <class S>
+__global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
+                                                 const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
+                                                 const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                                 const int* __restrict__ nb2, const uint32_t* __restrict__ slots, AmrGeo G,
+                                                 float nqs, float dec) {
+  __shared__ FillLds<typename S::real> L;
+  fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, blockIdx.x, threadIdx.x, 256, L);
 }
 // prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
 // non-equilibrium regularised and rescaled by nqs.
@@ -286,31 +286,20 @@ hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, cons
   return hipGetLastError();
 }
 
-hipError_t amr_prep(int prec, int level, const AmrGeo& G, const void* f, void* prep, const int* list, int n, float nqs,
-                    float dec, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  if (level == 0) {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_prep<S, true>), dim3(n), dim3(512), 0, st, RAW(f), WREAL(prep), list, G, nqs, dec));
-  } else {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_prep<S, false>), dim3(n), dim3(512), 0, st, RAW(f), WREAL(prep), list, G, nqs, dec));
-  }
-  return hipGetLastError();
-}
-
-hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* prep0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* ref0, const uint32_t* slots, int nf1, hipStream_t st) {
+hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
+                       const int* nb1, const uint32_t* slots, int nf1, float nqs, float dec, hipStream_t st) {
   if (nf1 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l1<typename S::real>), dim3(nf1), dim3(256), 0, st, REAL(prep0), WREAL(gh1),
-                                       l1pcb, l1oct, ref0, slots, G));
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l1<S>), dim3(nf1), dim3(256), 0, st, RAW(c0), WREAL(gh1), l1pcb, l1oct, nb1,
+                                       slots, G, nqs, dec));
   return hipGetLastError();
 }
 
-hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* ref1, const uint32_t* slots, int nf2,
-                       hipStream_t st) {
+hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, int nf2, float nqs,
+                       float dec, hipStream_t st) {
   if (nf2 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l2<typename S::real>), dim3(nf2), dim3(256), 0, st, REAL(prep1), WREAL(gh2),
-                                       l2pl1, l2oct, f1pos, l1grid, ref1, slots, G));
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l2<S>), dim3(nf2), dim3(256), 0, st, RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos,
+                                       l1grid, nb2, slots, G, nqs, dec));
   return hipGetLastError();
 }
 

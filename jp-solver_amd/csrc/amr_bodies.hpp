@@ -141,19 +141,9 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
   collide_store<S>(fin, om, fb, b, cell, enc, keep, dec);
 }
 
-// prep = feq + nqs * regularize(f - feq) (src/tgv3d_amr.cu:160-171), listed tiles only.
-template <class S, bool kDense>
-__device__ inline void prep_cell(const typename S::raw* __restrict__ f, typename S::real* __restrict__ pr,
-                                 const AmrGeo& G, float nqs, float dec, int b, int cell) {
-  using R = typename S::real;
-  R v[Q], nq[Q];
-  uint32_t di = 0;
-  if constexpr (kDense) {
-    di = dense_of_tile(G, b, cell);
-    load_dense<S>(f, G.g0.plane, di, dec, v);
-  } else {
-    load_tile<S>(f, b, cell, dec, v);
-  }
+// prep = feq + nqs * regularize(f - feq) of one cell, in place (src/tgv3d_amr.cu:160-171).
+template <class R> __device__ inline void prep_values(R* v, float nqs) {
+  R nq[Q];
   const Mom<R> m = moments(v);
   const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
   const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
@@ -164,79 +154,119 @@ __device__ inline void prep_cell(const typename S::raw* __restrict__ f, typename
   regularize(nq);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const R val = feq<q>(e) + R(nqs) * nq[q];
-    if constexpr (kDense) pr[uint64_t(q) * G.g0.plane + di] = val;
-    else pr[tidx(b, q, cell)] = val;
+    v[q] = feq<q>(e) + R(nqs) * nq[q];
   });
 }
 
-// Ghost record of level-1 tiles from prep0 (src/tgv3d_amr.cu:122-132, 173-186): for each slot whose
-// source position lies in an unrefined coarse tile, the trilinear interpolation of that direction.
-template <class R>
-__device__ inline void fill_l1_tile(const R* __restrict__ prep0, R* __restrict__ gh1, const int* __restrict__ l1pcb,
-                                    const int* __restrict__ l1oct, const int* __restrict__ ref0,
-                                    const uint32_t* __restrict__ slots, const AmrGeo& G, int fb, int tid, int nthr) {
+// Ghost fill of one fine tile (src/tgv3d_amr.cu:120-202). The trilinear corners of every ghost source of
+// a tile at fine origin o lie in the 6^3 parent-level box starting at o/2 - 1 (fine coordinate g has
+// parent base floor((g - 1)/2) for g in [o - 1, o + 8]). The workgroup stages that box into LDS with the
+// prep values computed in place (feq + nqs * regularize(f - feq) of the parent populations, exactly
+// what the reference's prep pass stores for these cells), then interpolates each slot from LDS. A slot
+// whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped.
+constexpr int kBox = 6, kBox3 = kBox * kBox * kBox;
+template <class R> struct FillLds {
+  R box[Q * kBox3];
+  int nb[kNbr];
+  int ok[kBox3];  // level 2: the parent cell exists (level-1 tiles are scattered)
+};
+__device__ inline void slot_source(uint32_t e, int& q, int& si, int& sj, int& sk, int& dir) {
+  q = (e >> 9) & 31;
+  si = int((e >> 14) & 15) - 1;
+  sj = int((e >> 18) & 15) - 1;
+  sk = int((e >> 22) & 15) - 1;
+  const int dx = si < 0 ? -1 : (si > 7 ? 1 : 0), dy = sj < 0 ? -1 : (sj > 7 ? 1 : 0), dz = sk < 0 ? -1 : (sk > 7 ? 1 : 0);
+  dir = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+}
+
+// Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists).
+template <class S>
+__device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
+                                    const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
+                                    const int* __restrict__ nb1, const uint32_t* __restrict__ slots, const AmrGeo& G,
+                                    float nqs, float dec, int fb, int tid, int nthr, FillLds<typename S::real>& L) {
+  using R = typename S::real;
   const int cb = l1pcb[fb], oct = l1oct[fb];
   const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
   const int ox = cbx * 2 * MB + (oct & 1) * MB, oy = cby * 2 * MB + ((oct >> 1) & 1) * MB, oz = cbz * 2 * MB + (oct >> 2) * MB;
+  const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
+  __syncthreads();  // a previous tile of this workgroup is done with the LDS
+  if (tid < kNbr) L.nb[tid] = nb1[size_t(fb) * kNbr + tid];
+  for (int c = tid; c < kBox3; c += nthr) {
+    const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
+    R v[Q];
+    load_dense<S>(c0, G.g0.plane, dense_cell(G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0)), dec, v);
+    prep_values(v, nqs);
+    sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
+  }
+  __syncthreads();
   for (int s = tid; s < kGhostSlots; s += nthr) {
-    const uint32_t e = slots[s];
-    const int q = (e >> 9) & 31;
-    const int gfi = pmod(ox + int((e >> 14) & 15) - 1, G.N1), gfj = pmod(oy + int((e >> 18) & 15) - 1, G.N1),
-              gfk = pmod(oz + int((e >> 22) & 15) - 1, G.N1);
-    const int scb = ((gfk / (2 * MB)) * G.cgx + (gfj / (2 * MB))) * G.cgx + (gfi / (2 * MB));
-    if (ref0[scb]) continue;
+    int q, si, sj, sk, dir;
+    slot_source(slots[s], q, si, sj, sk, dir);
+    if (L.nb[dir] >= 0) continue;
     int ib, jb, kb;
     float tx, ty, tz;
-    tri(gfi, ib, tx);
-    tri(gfj, jb, ty);
-    tri(gfk, kb, tz);
-    const R* __restrict__ p = prep0 + uint64_t(q) * G.g0.plane;
+    tri(ox + si, ib, tx);
+    tri(oy + sj, jb, ty);
+    tri(oz + sk, kb, tz);
+    const R* bq = L.box + q * kBox3 + ((kb - bz0) * kBox + (jb - by0)) * kBox + (ib - bx0);
     R out = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
       const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
-      const uint32_t idx = dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0));
-      out += R(wgt) * p[idx];
+      out += R(wgt) * bq[(dk * kBox + dj) * kBox + di];
     }
     gh1[size_t(fb) * kGhostSlots + s] = out;
   }
 }
 
-// Ghost record of level-2 tiles from prep1 over the scattered level 1, dropped corners renormalised
-// (src/tgv3d_amr.cu:133-144, 187-202).
-template <class R>
-__device__ inline void fill_l2_tile(const R* __restrict__ prep1, R* __restrict__ gh2, const int* __restrict__ l2pl1,
-                                    const int* __restrict__ l2oct, const int* __restrict__ f1pos,
-                                    const int* __restrict__ l1grid, const int* __restrict__ ref1,
-                                    const uint32_t* __restrict__ slots, const AmrGeo& G, int b, int tid, int nthr) {
+// Level 2 from the scattered level 1: box cells without a level-1 tile are dropped and the remaining
+// weights renormalised (src/tgv3d_amr.cu:133-144, 187-202).
+template <class S>
+__device__ inline void fill_l2_tile(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
+                                    const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
+                                    const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                    const int* __restrict__ nb2, const uint32_t* __restrict__ slots, const AmrGeo& G,
+                                    float nqs, float dec, int b, int tid, int nthr, FillLds<typename S::real>& L) {
+  using R = typename S::real;
   const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
   const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
   const int ox = px * 2 * MB + (oct & 1) * MB, oy = py * 2 * MB + ((oct >> 1) & 1) * MB, oz = pz * 2 * MB + (oct >> 2) * MB;
+  const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
+  __syncthreads();  // a previous tile of this workgroup is done with the LDS
+  if (tid < kNbr) L.nb[tid] = nb2[size_t(b) * kNbr + tid];
+  for (int c = tid; c < kBox3; c += nthr) {
+    const int ci = pmod(bx0 + c % kBox, G.N1), cj = pmod(by0 + (c / kBox) % kBox, G.N1), ck = pmod(bz0 + c / (kBox * kBox), G.N1);
+    const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
+    L.ok[c] = lb >= 0;
+    if (lb < 0) continue;
+    R v[Q];
+    load_tile<S>(f1, lb, ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7), dec, v);
+    prep_values(v, nqs);
+    sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
+  }
+  __syncthreads();
   for (int s = tid; s < kGhostSlots; s += nthr) {
-    const uint32_t e = slots[s];
-    const int q = (e >> 9) & 31;
-    const int g2i = pmod(ox + int((e >> 14) & 15) - 1, G.N2), g2j = pmod(oy + int((e >> 18) & 15) - 1, G.N2),
-              g2k = pmod(oz + int((e >> 22) & 15) - 1, G.N2);
-    const int lbs = l1grid[((g2k / (2 * MB)) * G.l1gx + (g2j / (2 * MB))) * G.l1gx + (g2i / (2 * MB))];
-    if (lbs >= 0 && ref1[lbs]) continue;
+    int q, si, sj, sk, dir;
+    slot_source(slots[s], q, si, sj, sk, dir);
+    if (L.nb[dir] >= 0) continue;
     int ib, jb, kb;
     float tx, ty, tz, wsum = 0.f;
-    tri(g2i, ib, tx);
-    tri(g2j, jb, ty);
-    tri(g2k, kb, tz);
+    tri(ox + si, ib, tx);
+    tri(oy + sj, jb, ty);
+    tri(oz + sk, kb, tz);
+    const int base = ((kb - bz0) * kBox + (jb - by0)) * kBox + (ib - bx0);
+    const R* bq = L.box + q * kBox3 + base;
     R out = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
       const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
-      const int ci = pmod(ib + di, G.N1), cj = pmod(jb + dj, G.N1), ck = pmod(kb + dk, G.N1);
-      const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
-      if (lb < 0) continue;
-      const int cc = ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7);
+      const int o = (dk * kBox + dj) * kBox + di;
+      if (!L.ok[base + o]) continue;
       wsum += wgt;
-      out += R(wgt) * prep1[tidx(lb, q, cc)];
+      out += R(wgt) * bq[o];
     }
     if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
     gh2[size_t(b) * kGhostSlots + s] = out;

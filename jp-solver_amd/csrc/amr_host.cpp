@@ -107,7 +107,6 @@ static int alloc_l1_pool(jp_ctx* c, bool with_f1a) {
   if (with_f1a)
     if (int rc = ctx_alloc(c, &a.f1a, size_t(a.cap1) * tile * e, "f1a")) return rc;
   if (int rc = ctx_alloc(c, &a.f1b, size_t(a.cap1) * tile * e, "f1b")) return rc;
-  if (int rc = ctx_alloc(c, &a.prep1, size_t(a.cap1) * tile * r, "prep1")) return rc;
   if (int rc = ctx_alloc(c, &a.gh1, size_t(a.cap1) * kGhostSlots * r, "gh1")) return rc;
   return JP_OK;
 }
@@ -125,7 +124,7 @@ static int grow_l1(jp_ctx* c, bool keep_f1a) {
     return fail(c, JP_ERR_CAPACITY, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
                                         std::to_string(a.cap1) + " and the budget (threshold ties); raise --maxgb");
   HCK(c, hipStreamSynchronize(c->st));
-  for (void* p : {a.f1b, a.prep1, a.gh1}) ctx_free(c, p);
+  for (void* p : {a.f1b, a.gh1}) ctx_free(c, p);
   if (!keep_f1a) ctx_free(c, a.f1a);
   a.cap1 = 8 * cn;
   a.f1_stale = keep_f1a;
@@ -321,16 +320,13 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
   HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
   std::swap(c->c0a, c->c0b);
   if (!a.nf1) return JP_OK;
-  if (!a.skipprep) {
-    HCK(c, amr_prep(p, 0, G, c->c0a, a.prep0, a.skin0, a.n_skin0, float(c->d.nqs01), dec(c, 0), st));
-    HCK(c, amr_fill_l1(p, G, a.prep0, a.gh1, a.l1pcb, a.l1oct, a.ref0, a.slots, a.nf1, st));
-  }
+  if (!a.skipprep)  // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
+    HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, a.nf1, float(c->d.nqs01), dec(c, 0), st));
   for (int s1 = 0; s1 < 2; ++s1) {
     if (a.nf2) {
-      if (!a.skipprep) {
-        HCK(c, amr_prep(p, 1, G, a.f1a, a.prep1, a.skin1, a.n_skin1, float(c->d.nqs12), dec(c, 1), st));
-        HCK(c, amr_fill_l2(p, G, a.prep1, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.slots, a.nf2, st));
-      }
+      if (!a.skipprep)
+        HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, a.nf2,
+                           float(c->d.nqs12), dec(c, 1), st));
       // the second level-2 substep carries the restriction into level 1 in its epilogue
       HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
       std::swap(a.f2a, a.f2b);
@@ -409,11 +405,10 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
   // pools
   c->g0 = make_dense(N, 0);
   c->variant = default_variant(c->prec);
-  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec);
+  const size_t e = prec_bytes(c->prec);
   const size_t dense_b = size_t(Q) * c->g0.plane;
   if (int rc = ctx_alloc(c, &c->c0a, dense_b * e, "c0a")) return rc;
   if (int rc = ctx_alloc(c, &c->c0b, dense_b * e, "c0b")) return rc;
-  if (int rc = ctx_alloc(c, &a.prep0, dense_b * r, "prep0")) return rc;
   if (int rc = alloc_l1_pool(c, true)) return rc;
   // level-1 tables are sized for full refinement (8 cnb0 tiles) so that only the pools ever grow
   const size_t n1 = size_t(8) * a.cnb0, n01 = n1;

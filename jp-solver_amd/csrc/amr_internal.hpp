@@ -50,14 +50,13 @@ hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const vo
 hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
                                   const int* nb, int ntiles, double omega, float enc, float dec, void* cf, const int* ptile,
                                   const int* poct, float nqi, float cenc, hipStream_t st);
-// prep = feq + nqs * regularize(f - feq) over the listed tiles; level 0 reads/writes dense planes
-hipError_t amr_prep(int prec, int level, const AmrGeo& G, const void* f, void* prep, const int* list, int n,
-                    float nqs, float dec, hipStream_t st);
-hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* prep0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* ref0, const uint32_t* slots, int nf1, hipStream_t st);
-hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* prep1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* ref1, const uint32_t* slots, int nf2,
-                       hipStream_t st);
+// Ghost records of level 1 (from the dense level 0 c0) and level 2 (from level 1 f1): prep values of
+// the parent box computed on the fly, trilinear interpolation of the slots whose source tile is absent.
+hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
+                       const int* nb1, const uint32_t* slots, int nf1, float nqs, float dec, hipStream_t st);
+hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, int nf2, float nqs,
+                       float dec, hipStream_t st);
 hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct,
                        int nf1, float nqs, float dec0, float enc1, hipStream_t st);
 hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,

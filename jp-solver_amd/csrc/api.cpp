@@ -47,6 +47,9 @@ void ctx_free(jp_ctx* c, void* p) {
   (void)hipFree(p);
 }
 
+// Elements appended to each population plane by default.
+uint64_t default_plane_pad(int, int) { return 0; }
+
 double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 / u0))); }
 // Dense-step launch variant per storage form, from the 512^3 sweep on MI355X: FP32 8 cells per lane in
 // 512-lane groups with cached loads and stores; 16-bit forms one cell per lane, cached loads, streaming
@@ -128,7 +131,9 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
   const double s = (p->prec == JP_PREC_FP16 || p->prec == JP_PREC_BF16) ? (p->scale > 0 ? p->scale : default_scale(p->u0)) : 1.0;
   if (s != std::exp2(std::round(std::log2(s)))) return fail(c, JP_ERR_ARG, "scale must be a power of two");
   c->d.scale[0] = c->d.scale[1] = c->d.scale[2] = s;
-  c->g0 = make_dense(N, 0);
+  if (p->plane_pad < -1 || (p->plane_pad > 0 && p->plane_pad % 64 != 0))
+    return fail(c, JP_ERR_ARG, "plane_pad must be a non-negative multiple of 64, 0 (default) or -1 (none)");
+  c->g0 = make_dense(N, p->plane_pad > 0 ? uint64_t(p->plane_pad) : (p->plane_pad < 0 ? 0 : default_plane_pad(N, c->prec)));
   const size_t bytes = size_t(Q) * c->g0.plane * prec_bytes(c->prec);
   if (int r = ctx_alloc(c, &c->c0a, bytes, "populations a")) return r;
   if (int r = ctx_alloc(c, &c->c0b, bytes, "populations b")) return r;

@@ -25,7 +25,7 @@ struct AmrState {
   int cap2_alloc = 0;  // level-2 tiles the allocated pool holds
   int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)  // f1b still has a pre-growth size (until the regrid's migration completes)
   void *f1a = nullptr, *f1b = nullptr, *f2a = nullptr, *f2b = nullptr;  // stored populations
-  void *prep0 = nullptr, *prep1 = nullptr, *gh1 = nullptr, *gh2 = nullptr;  // working-type values
+  void *gh1 = nullptr, *gh2 = nullptr;  // ghost records, working-type values
   int *ref0 = nullptr, *fof0 = nullptr, *rl0 = nullptr, *l1pcb = nullptr, *l1oct = nullptr, *f1pos = nullptr;
   int *l1grid = nullptr, *ref1 = nullptr, *fof1 = nullptr, *rl1 = nullptr, *l2pl1 = nullptr, *l2oct = nullptr;
   int *skin0 = nullptr, *skin1 = nullptr, *o_ref0 = nullptr, *o_fof0 = nullptr, *o_l1grid = nullptr;

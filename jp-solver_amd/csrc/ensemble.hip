@@ -27,32 +27,21 @@ __global__ __launch_bounds__(256) void kb_dense_step(const MemberDesc* __restric
                        uint32_t(w.y) * 256 + threadIdx.x);
   }
 }
-// items: (member, index into the skin list); level 0 prep reads the dense lattice
+// items: (member, fine tile); p: parity of the parent level's current buffer
 template <class S, int kLevel>
-__global__ __launch_bounds__(512) void kb_prep(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
+__global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
                                                int p) {
   using R = typename S::real;
-  for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
-    const MemberDesc& d = md[w.x];
-    if constexpr (kLevel == 0)
-      prep_cell<S, true>(cr<S>(d.c0[p]), static_cast<R*>(d.prep0), d.G, d.nqs01, d.dec[0], d.skin0[w.y], threadIdx.x);
-    else
-      prep_cell<S, false>(cr<S>(d.f1[p]), static_cast<R*>(d.prep1), d.G, d.nqs12, d.dec[1], d.skin1[w.y], threadIdx.x);
-  }
-}
-// items: (member, fine tile)
-template <class R, int kLevel>
-__global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n) {
+  __shared__ FillLds<R> L;
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
     if constexpr (kLevel == 1)
-      fill_l1_tile<R>(static_cast<const R*>(d.prep0), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.ref0, d.slots, d.G, w.y,
-                      threadIdx.x, 256);
+      fill_l1_tile<S>(cr<S>(d.c0[p]), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.nb1, d.slots, d.G, d.nqs01, d.dec[0], w.y,
+                      threadIdx.x, 256, L);
     else
-      fill_l2_tile<R>(static_cast<const R*>(d.prep1), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.ref1,
-                      d.slots, d.G, w.y, threadIdx.x, 256);
+      fill_l2_tile<S>(cr<S>(d.f1[p]), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.nb2, d.slots, d.G,
+                      d.nqs12, d.dec[1], w.y, threadIdx.x, 256, L);
   }
 }
 // items: (member, fine tile)
@@ -120,14 +109,8 @@ hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* i
   const dim3 g = grid_of(n, groups);
   switch (k) {
     case EK_DENSE: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_dense_step<S>), g, dim3(256), 0, st, md, items, n, pa)); break;
-    case EK_PREP0: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_prep<S, 0>), g, dim3(512), 0, st, md, items, n, pa)); break;
-    case EK_PREP1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_prep<S, 1>), g, dim3(512), 0, st, md, items, n, pa)); break;
-    case EK_FILL1:
-      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<typename S::real, 1>), g, dim3(256), 0, st, md, items, n));
-      break;
-    case EK_FILL2:
-      JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<typename S::real, 2>), g, dim3(256), 0, st, md, items, n));
-      break;
+    case EK_FILL1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<S, 1>), g, dim3(256), 0, st, md, items, n, pa)); break;
+    case EK_FILL2: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fill<S, 2>), g, dim3(256), 0, st, md, items, n, pa)); break;
     case EK_STEP1: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 1>), g, dim3(512), 0, st, md, items, n, pa)); break;
     case EK_STEP2: JP_DISPATCH(prec, hipLaunchKernelGGL((kb_fine_step<S, 2>), g, dim3(512), 0, st, md, items, n, pa)); break;
     case EK_STEP1R:

@@ -1,5 +1,6 @@
 // Batched ensemble: per-member device descriptor and the batched launch interface (ensemble.hip).
 #pragma once
+#include "../../include/jp.h"
 #include "amr_internal.hpp"
 
 namespace jp {
@@ -10,9 +11,9 @@ struct MemberDesc {
   void* c0[2];
   void* f1[2];
   void* f2[2];
-  void *prep0, *prep1, *gh1, *gh2;
+  void *gh1, *gh2;
   const int *ref0, *fof0, *rl0, *l1pcb, *l1oct, *f1pos, *l1grid, *ref1, *fof1, *rl1, *l2pl1, *l2oct;
-  const int *skin0, *skin1, *nb1, *nb2;
+  const int *nb1, *nb2;
   const uint32_t* slots;
   Dense g0;
   AmrGeo G;
@@ -21,8 +22,10 @@ struct MemberDesc {
   float nqs01, nqi10, nqs12, nqi21;
 };
 
-// EK_STEP2R / EK_STEP1R: the second substep of a fine level with the restriction into its parent fused in.
-enum EnsKernel { EK_DENSE = 0, EK_PREP0, EK_FILL1, EK_PREP1, EK_FILL2, EK_STEP2, EK_STEP2R, EK_STEP1, EK_STEP1R, EK_COUNT };
+// EK_FILL1 / EK_FILL2: ghost records (prep of the parent level computed in the fill). EK_STEP2R /
+// EK_STEP1R: the second substep of a fine level with the restriction into its parent fused in.
+enum EnsKernel { EK_DENSE = 0, EK_FILL1, EK_FILL2, EK_STEP2, EK_STEP2R, EK_STEP1, EK_STEP1R, EK_COUNT };
+static_assert(EK_COUNT == JP_ENS_KERNELS, "one tuned launch shape per batched kernel");
 
 // One batched launch of kernel k over n items with at most `groups` workgroups. pa: parity of the
 // buffer read, pb: the parent level's parity (the fused restriction's target; EK_STEP2R / EK_STEP1R only).

@@ -40,8 +40,6 @@ static MemberDesc describe(const jp_ctx* m) {
   d.f1[1] = a.f1b;
   d.f2[0] = a.f2a;
   d.f2[1] = a.f2b;
-  d.prep0 = a.prep0;
-  d.prep1 = a.prep1;
   d.gh1 = a.gh1;
   d.gh2 = a.gh2;
   d.ref0 = a.ref0;
@@ -56,8 +54,6 @@ static MemberDesc describe(const jp_ctx* m) {
   d.rl1 = a.rl1;
   d.l2pl1 = a.l2pl1;
   d.l2oct = a.l2oct;
-  d.skin0 = a.skin0;
-  d.skin1 = a.skin1;
   d.nb1 = a.nb1;
   d.nb2 = a.nb2;
   d.slots = a.slots;
@@ -105,13 +101,11 @@ static int upload(jp_ctx* c) {
     if (m->kind != 1) continue;
     const AmrState& a = m->amr;
     if (a.nf1) {
-      for (int k = 0; k < a.n_skin0; ++k) L[EK_PREP0].push_back(make_int2(i, k));
       for (int b = 0; b < a.nf1; ++b) L[EK_FILL1].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf1; ++b) L[EK_STEP1].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf1; ++b) L[EK_STEP1R].push_back(make_int2(i, b));
     }
     if (a.nf1 && a.nf2) {
-      for (int k = 0; k < a.n_skin1; ++k) L[EK_PREP1].push_back(make_int2(i, k));
       for (int b = 0; b < a.nf2; ++b) L[EK_FILL2].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf2; ++b) L[EK_STEP2].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf2; ++b) L[EK_STEP2R].push_back(make_int2(i, b));
@@ -169,11 +163,9 @@ int ens_step_once(jp_ctx* c) {
   };
   HCK(c, L(EK_DENSE, e.p0, 0));
   swap_all(e, 0);
-  HCK(c, L(EK_PREP0, e.p0, 0));
-  HCK(c, L(EK_FILL1, 0, 0));
+  HCK(c, L(EK_FILL1, e.p0, 0));
   for (int s1 = 0; s1 < 2; ++s1) {
-    HCK(c, L(EK_PREP1, e.p1, 0));
-    HCK(c, L(EK_FILL2, 0, 0));
+    HCK(c, L(EK_FILL2, e.p1, 0));
     HCK(c, L(EK_STEP2, e.p2, 0));
     swap_all(e, 2);
     HCK(c, L(EK_STEP2R, e.p2, e.p1));  // second level-2 substep, restricted into level 1
@@ -222,8 +214,8 @@ double ens_work(const jp_ctx* c) {
 }
 
 // Each candidate launch re-runs one batched kernel on unchanged inputs: the dense and fine steps write
-// the non-current buffer, prep and fill their scratch, level-1 restriction the values it last wrote.
-// Level-2 restriction is not idempotent at the end of a step, so it takes level-1's choice.
+// the non-current buffer, the fills their ghost records. The fused step + restriction launches also
+// write the parent level's current buffer, so they are not timed and take their plain step's choice.
 int ens_autotune(jp_ctx* c, int* out) {
   EnsState& e = *c->ens;
   if (int r = ens_check(c)) return r;
@@ -240,7 +232,7 @@ int ens_autotune(jp_ctx* c, int* out) {
       e.groups[k] = cu * 4;
       continue;
     }
-    const int pa = (k == EK_DENSE || k == EK_PREP0) ? e.p0 : (k == EK_STEP2) ? e.p2 : e.p1;
+    const int pa = (k == EK_DENSE || k == EK_FILL1) ? e.p0 : (k == EK_STEP2) ? e.p2 : e.p1;
     const int pb = 0;
     float best = 1e30f;
     int bestg = cu * 4;

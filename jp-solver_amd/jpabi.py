@@ -29,7 +29,8 @@ class JPError(RuntimeError):
 
 class UniformParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("re", ctypes.c_double), ("u0", ctypes.c_double), ("prec", ctypes.c_int),
-                ("phase", ctypes.c_double * 3), ("scale", ctypes.c_double), ("device", ctypes.c_int)]
+                ("phase", ctypes.c_double * 3), ("scale", ctypes.c_double), ("device", ctypes.c_int),
+                ("plane_pad", ctypes.c_long)]
 
 
 class AmrParams(ctypes.Structure):
@@ -40,6 +41,7 @@ class AmrParams(ctypes.Structure):
                 ("scale", ctypes.c_double * 3), ("device", ctypes.c_int)]
 
 
+ENS_KERNELS = 7  # JP_ENS_KERNELS
 COUNT_NAMES = ["cnb0", "nref0", "nf1", "nref1", "nf2", "n_skin0", "n_skin1", "cap_nref0", "cap1", "cap_nref1",
                "cap2", "hw_nf1", "hw_nf2", "ndeep"]
 
@@ -206,9 +208,10 @@ class _Ctx:
 class Uniform(_Ctx):
     """Uniform periodic D3Q19 TGV lattice on the GPU (include/jp.h jp_uniform_create)."""
 
-    def __init__(self, n, re=1600.0, u0=0.05, prec=PREC_FP32, phase=(0.0, 0.0, 0.0), scale=0.0, device=0):
+    def __init__(self, n, re=1600.0, u0=0.05, prec=PREC_FP32, phase=(0.0, 0.0, 0.0), scale=0.0, device=0,
+                 plane_pad=0):
         self.prec, self.n = prec, n
-        p = UniformParams(n=n, re=re, u0=u0, prec=prec, scale=scale, device=device)
+        p = UniformParams(n=n, re=re, u0=u0, prec=prec, scale=scale, device=device, plane_pad=plane_pad)
         for k in range(3):
             p.phase[k] = phase[k]
         h = ctypes.c_void_p()
@@ -326,6 +329,6 @@ class Ensemble(_Ctx):
         self._check(lib().jp_use_graph(self.h, 1 if on else 0))
 
     def autotune(self):
-        g = (ctypes.c_int * 9)()
+        g = (ctypes.c_int * ENS_KERNELS)()
         self._check(lib().jp_autotune(self.h, g))
         return list(g)
