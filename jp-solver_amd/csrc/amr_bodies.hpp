@@ -163,13 +163,26 @@ template <class R> __device__ inline void prep_values(R* v, float nqs) {
 // parent base floor((g - 1)/2) for g in [o - 1, o + 8]). The workgroup stages that box into LDS with the
 // prep values computed in place (feq + nqs * regularize(f - feq) of the parent populations, exactly
 // what the reference's prep pass stores for these cells), then interpolates each slot from LDS. A slot
-// whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped.
+// whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped, and only the
+// box cells that some remaining slot reads are staged (a tile with all 26 neighbours stages nothing).
 constexpr int kBox = 6, kBox3 = kBox * kBox * kBox;
 template <class R> struct FillLds {
   R box[Q * kBox3];
   int nb[kNbr];
   int ok[kBox3];  // level 2: the parent cell exists (level-1 tiles are scattered)
 };
+// Whether box cell (li, lj, lk) is a trilinear corner of some ghost slot. Slots toward a neighbour
+// offset d reach box coordinates {0, 1} for d = -1, {4, 5} for d = +1 and 0..5 for d = 0 (per axis); a
+// slot is a ghost only when that neighbour tile is absent (nb < 0).
+__device__ inline bool box_cell_needed(const int* nb, int li, int lj, int lk) {
+  const int xl = li < 2 ? -1 : 0, xh = li > 3 ? 1 : 0, yl = lj < 2 ? -1 : 0, yh = lj > 3 ? 1 : 0;
+  const int zl = lk < 2 ? -1 : 0, zh = lk > 3 ? 1 : 0;
+  for (int dz = zl; dz <= zh; ++dz)
+    for (int dy = yl; dy <= yh; ++dy)
+      for (int dx = xl; dx <= xh; ++dx)
+        if (nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] < 0) return true;
+  return false;
+}
 __device__ inline void slot_source(uint32_t e, int& q, int& si, int& sj, int& sk, int& dir) {
   q = (e >> 9) & 31;
   si = int((e >> 14) & 15) - 1;
@@ -192,8 +205,10 @@ __device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, type
   const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb1[size_t(fb) * kNbr + tid];
+  __syncthreads();
   for (int c = tid; c < kBox3; c += nthr) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
+    if (!box_cell_needed(L.nb, li, lj, lk)) continue;
     R v[Q];
     load_dense<S>(c0, G.g0.plane, dense_cell(G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0)), dec, v);
     prep_values(v, nqs);
@@ -236,11 +251,15 @@ __device__ inline void fill_l2_tile(const typename S::raw* __restrict__ f1, type
   const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb2[size_t(b) * kNbr + tid];
+  __syncthreads();
   for (int c = tid; c < kBox3; c += nthr) {
-    const int ci = pmod(bx0 + c % kBox, G.N1), cj = pmod(by0 + (c / kBox) % kBox, G.N1), ck = pmod(bz0 + c / (kBox * kBox), G.N1);
+    const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
+    L.ok[c] = 0;
+    if (!box_cell_needed(L.nb, li, lj, lk)) continue;
+    const int ci = pmod(bx0 + li, G.N1), cj = pmod(by0 + lj, G.N1), ck = pmod(bz0 + lk, G.N1);
     const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
-    L.ok[c] = lb >= 0;
     if (lb < 0) continue;
+    L.ok[c] = 1;
     R v[Q];
     load_tile<S>(f1, lb, ((ck & 7) * MB + (cj & 7)) * MB + (ci & 7), dec, v);
     prep_values(v, nqs);
