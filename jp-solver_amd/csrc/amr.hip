@@ -15,7 +15,7 @@ __global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __rest
                                                    const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                                    typename S::real om, float enc, float dec) {
   __shared__ int snb[kNbr];
-  const int b = blockIdx.x;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
   __syncthreads();
   fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x);
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __re
                                                      float nqi, float cenc) {
   __shared__ int snb[kNbr];
   __shared__ typename S::real keep[Q * MB3];
-  const int b = blockIdx.x;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
   __syncthreads();
   fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x, keep);
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void k_fill_l1(const typename S::raw* __restri
                                                  const int* __restrict__ nb1, const uint32_t* __restrict__ slots, AmrGeo G,
                                                  float nqs, float dec) {
   __shared__ FillLds<typename S::real> L;
-  fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, blockIdx.x, threadIdx.x, 256, L);
+  fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, L);
 }
 template <class S>
 __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
@@ -53,7 +53,8 @@ __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restri
                                                  const int* __restrict__ nb2, const uint32_t* __restrict__ slots, AmrGeo G,
                                                  float nqs, float dec) {
   __shared__ FillLds<typename S::real> L;
-  fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, blockIdx.x, threadIdx.x, 256, L);
+  fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, xcd_tile(blockIdx.x, gridDim.x),
+                  threadIdx.x, L);
 }
 // prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
 // non-equilibrium regularised and rescaled by nqs.
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(512) void k_seed_l1(const typename S::raw* __restri
                                                  const int* __restrict__ o_ref0, const int* __restrict__ o_fof0,
                                                  const typename S::raw* __restrict__ oldf1, AmrGeo G, float nqs,
                                                  float dec0, float enc1) {
-  const int fb = blockIdx.x, cell = threadIdx.x;
+  const int fb = xcd_tile(blockIdx.x, gridDim.x), cell = threadIdx.x;
   const int cb = l1pcb[fb], oct = l1oct[fb];
   if (o_ref0 && o_ref0[cb]) {  // migration: the parent was refined before, copy its old tile
     const int ofb = o_fof0[cb] + oct;
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(512) void k_seed_l2(const typename S::raw* __restri
                                                  const int* __restrict__ o_l1grid, const int* __restrict__ o_ref1,
                                                  const int* __restrict__ o_fof1, const typename S::raw* __restrict__ oldf2,
                                                  AmrGeo G, float nqs, float dec1, float enc2) {
-  const int b = blockIdx.x, cell = threadIdx.x;
+  const int b = xcd_tile(blockIdx.x, gridDim.x), cell = threadIdx.x;
   const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
   if (o_l1grid) {  // migration keyed by the level-1 grid position
     const int ol = o_l1grid[p];

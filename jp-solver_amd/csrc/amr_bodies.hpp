@@ -31,6 +31,16 @@ __device__ inline uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
 }
 __device__ inline size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
 
+// Tile of workgroup w in a launch of n one-tile workgroups. The dispatcher deals workgroups to the 8
+// XCDs round-robin (w -> XCD w % 8), and each XCD has its own L2: giving every XCD a contiguous run of
+// tiles keeps neighbouring tiles (the octants of one parent, whose faces and parent boxes overlap) in
+// one L2 instead of spreading them over all eight. A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int w, int n) {
+  const int x = w % kXcds, i = w / kXcds, q = n / kXcds, r = n % kXcds;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <class S> __device__ inline void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
                                                     typename S::real* out) {
   sfor<Q>([&](auto qc) {
@@ -165,7 +175,8 @@ template <class R> __device__ inline void prep_values(R* v, float nqs) {
 // what the reference's prep pass stores for these cells), then interpolates each slot from LDS. A slot
 // whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped, and only the
 // box cells that some remaining slot reads are staged (a tile with all 26 neighbours stages nothing).
-constexpr int kBox = 6, kBox3 = kBox * kBox * kBox;
+constexpr int kBox = kFillBox, kBox3 = kBox * kBox * kBox;
+constexpr int kFillThreads = 256, kSlotsPerLane = (kGhostSlots + kFillThreads - 1) / kFillThreads;
 template <class R> struct FillLds {
   R box[Q * kBox3];
   int nb[kNbr];
@@ -183,13 +194,10 @@ __device__ inline bool box_cell_needed(const int* nb, int li, int lj, int lk) {
         if (nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] < 0) return true;
   return false;
 }
-__device__ inline void slot_source(uint32_t e, int& q, int& si, int& sj, int& sk, int& dir) {
-  q = (e >> 9) & 31;
-  si = int((e >> 14) & 15) - 1;
-  sj = int((e >> 18) & 15) - 1;
-  sk = int((e >> 22) & 15) - 1;
-  const int dx = si < 0 ? -1 : (si > 7 ? 1 : 0), dy = sj < 0 ? -1 : (sj > 7 ? 1 : 0), dz = sk < 0 ? -1 : (sk > 7 ? 1 : 0);
-  dir = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+// trilinear weight of corner c (bits di, dj, dk) for a fill-table entry's source parities
+__device__ inline float corner_weight(uint32_t e, int c) {
+  const float tx = (e >> 18) & 1 ? 0.25f : 0.75f, ty = (e >> 19) & 1 ? 0.25f : 0.75f, tz = (e >> 20) & 1 ? 0.25f : 0.75f;
+  return ((c & 1) ? tx : 1 - tx) * ((c & 2) ? ty : 1 - ty) * ((c & 4) ? tz : 1 - tz);
 }
 
 // Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists).
@@ -197,16 +205,22 @@ template <class S>
 __device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                     const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                     const int* __restrict__ nb1, const uint32_t* __restrict__ slots, const AmrGeo& G,
-                                    float nqs, float dec, int fb, int tid, int nthr, FillLds<typename S::real>& L) {
+                                    float nqs, float dec, int fb, int tid, FillLds<typename S::real>& L) {
   using R = typename S::real;
   const int cb = l1pcb[fb], oct = l1oct[fb];
   const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
   const int ox = cbx * 2 * MB + (oct & 1) * MB, oy = cby * 2 * MB + ((oct >> 1) & 1) * MB, oz = cbz * 2 * MB + (oct >> 2) * MB;
   const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
+  uint32_t ent[kSlotsPerLane];  // this lane's slot entries, loaded ahead of the staging (0: none)
+#pragma unroll
+  for (int k = 0; k < kSlotsPerLane; ++k) {
+    const int s = tid + k * kFillThreads;
+    ent[k] = s < kGhostSlots ? slots[s] : 0u;
+  }
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb1[size_t(fb) * kNbr + tid];
   __syncthreads();
-  for (int c = tid; c < kBox3; c += nthr) {
+  for (int c = tid; c < kBox3; c += kFillThreads) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
     R v[Q];
@@ -215,24 +229,16 @@ __device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, type
     sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
   }
   __syncthreads();
-  for (int s = tid; s < kGhostSlots; s += nthr) {
-    int q, si, sj, sk, dir;
-    slot_source(slots[s], q, si, sj, sk, dir);
-    if (L.nb[dir] >= 0) continue;
-    int ib, jb, kb;
-    float tx, ty, tz;
-    tri(ox + si, ib, tx);
-    tri(oy + sj, jb, ty);
-    tri(oz + sk, kb, tz);
-    const R* bq = L.box + q * kBox3 + ((kb - bz0) * kBox + (jb - by0)) * kBox + (ib - bx0);
+#pragma unroll
+  for (int k = 0; k < kSlotsPerLane; ++k) {
+    if (!ent[k]) continue;
+    const uint32_t e = ent[k];
+    if (L.nb[(e >> 5) & 31] >= 0) continue;
+    const R* bq = L.box + (e & 31) * kBox3 + ((e >> 10) & 255);
     R out = R(0);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
-      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
-      out += R(wgt) * bq[(dk * kBox + dj) * kBox + di];
-    }
-    gh1[size_t(fb) * kGhostSlots + s] = out;
+    for (int c = 0; c < 8; ++c) out += R(corner_weight(e, c)) * bq[((c >> 2) * kBox + ((c >> 1) & 1)) * kBox + (c & 1)];
+    gh1[size_t(fb) * kGhostSlots + tid + k * kFillThreads] = out;
   }
 }
 
@@ -243,16 +249,22 @@ __device__ inline void fill_l2_tile(const typename S::raw* __restrict__ f1, type
                                     const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                     const int* __restrict__ f1pos, const int* __restrict__ l1grid,
                                     const int* __restrict__ nb2, const uint32_t* __restrict__ slots, const AmrGeo& G,
-                                    float nqs, float dec, int b, int tid, int nthr, FillLds<typename S::real>& L) {
+                                    float nqs, float dec, int b, int tid, FillLds<typename S::real>& L) {
   using R = typename S::real;
   const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
   const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
   const int ox = px * 2 * MB + (oct & 1) * MB, oy = py * 2 * MB + ((oct >> 1) & 1) * MB, oz = pz * 2 * MB + (oct >> 2) * MB;
   const int bx0 = ox / 2 - 1, by0 = oy / 2 - 1, bz0 = oz / 2 - 1;
+  uint32_t ent[kSlotsPerLane];  // this lane's slot entries, loaded ahead of the staging (0: none)
+#pragma unroll
+  for (int k = 0; k < kSlotsPerLane; ++k) {
+    const int s = tid + k * kFillThreads;
+    ent[k] = s < kGhostSlots ? slots[s] : 0u;
+  }
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb2[size_t(b) * kNbr + tid];
   __syncthreads();
-  for (int c = tid; c < kBox3; c += nthr) {
+  for (int c = tid; c < kBox3; c += kFillThreads) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     L.ok[c] = 0;
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
@@ -266,29 +278,25 @@ __device__ inline void fill_l2_tile(const typename S::raw* __restrict__ f1, type
     sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
   }
   __syncthreads();
-  for (int s = tid; s < kGhostSlots; s += nthr) {
-    int q, si, sj, sk, dir;
-    slot_source(slots[s], q, si, sj, sk, dir);
-    if (L.nb[dir] >= 0) continue;
-    int ib, jb, kb;
-    float tx, ty, tz, wsum = 0.f;
-    tri(ox + si, ib, tx);
-    tri(oy + sj, jb, ty);
-    tri(oz + sk, kb, tz);
-    const int base = ((kb - bz0) * kBox + (jb - by0)) * kBox + (ib - bx0);
-    const R* bq = L.box + q * kBox3 + base;
+#pragma unroll
+  for (int k = 0; k < kSlotsPerLane; ++k) {
+    if (!ent[k]) continue;
+    const uint32_t e = ent[k];
+    if (L.nb[(e >> 5) & 31] >= 0) continue;
+    const int base = (e >> 10) & 255;
+    const R* bq = L.box + (e & 31) * kBox3 + base;
+    float wsum = 0.f;
     R out = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
-      const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
-      const int o = (dk * kBox + dj) * kBox + di;
+      const int o = ((c >> 2) * kBox + ((c >> 1) & 1)) * kBox + (c & 1);
       if (!L.ok[base + o]) continue;
+      const float wgt = corner_weight(e, c);
       wsum += wgt;
       out += R(wgt) * bq[o];
     }
     if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
-    gh2[size_t(b) * kGhostSlots + s] = out;
+    gh2[size_t(b) * kGhostSlots + tid + k * kFillThreads] = out;
   }
 }
 

@@ -29,16 +29,21 @@ static size_t real_bytes(int prec) { return prec == JP_PREC_FP64 ? 8 : 4; }
 static float enc(const jp_ctx* c, int l) { return float(c->d.scale[l]); }
 static float dec(const jp_ctx* c, int l) { return float(1.0 / c->d.scale[l]); }
 
-// fill table of the compact ghost record: one entry per slot
+// Fill table of the compact ghost record, one entry per slot (encoding in amr_internal.hpp). Every fine
+// tile's origin is even on its own grid, so the box offset and the weights depend on the slot only.
 static std::vector<uint32_t> ghost_fill_table() {
-  std::vector<uint32_t> t(kGhostSlots, 0xffffffffu);
+  std::vector<uint32_t> t(kGhostSlots, 0u);
+  auto tri_local = [](int g) { return ((g & 1) ? (g >> 1) : (g >> 1) - 1) + 1; };  // parent base - (o/2 - 1)
+  auto side = [](int s) { return s < 0 ? -1 : (s > 7 ? 1 : 0); };
   for (int q = 1; q < Q; ++q)
     for (int cell = 0; cell < MB3; ++cell) {
       const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
       const int si = i - CX(q), sj = j - CY(q), sk = k - CZ(q);
       if (si >= 0 && si < 8 && sj >= 0 && sj < 8 && sk >= 0 && sk < 8) continue;
-      const int s = ghost_slot(q, i, j, k);
-      t[s] = uint32_t(cell) | uint32_t(q) << 9 | uint32_t(si + 1) << 14 | uint32_t(sj + 1) << 18 | uint32_t(sk + 1) << 22;
+      const int dir = (side(sk) + 1) * 9 + (side(sj) + 1) * 3 + side(si) + 1;
+      const int box = (tri_local(sk) * kFillBox + tri_local(sj)) * kFillBox + tri_local(si);
+      t[ghost_slot(q, i, j, k)] = uint32_t(q) | uint32_t(dir) << 5 | uint32_t(box) << 10 | uint32_t(si & 1) << 18 |
+                                  uint32_t(sj & 1) << 19 | uint32_t(sk & 1) << 20;
     }
   return t;
 }

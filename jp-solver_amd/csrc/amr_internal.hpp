@@ -12,8 +12,13 @@ namespace jp {
 constexpr int kGhostSlots = 1824;
 constexpr int kNbr = 27;
 
-// Fill-table entry of one ghost slot: cell | q << 9 | (si+1) << 14 | (sj+1) << 18 | (sk+1) << 22, where
-// (si, sj, sk) in [-1, 8] is the source position relative to the tile.
+// Ghost fills interpolate from the 6^3 parent-level box starting at parent coordinate o/2 - 1 of a tile
+// at fine origin o. Fill-table entry of one ghost slot, source (si, sj, sk) in [-1, 8]^3 relative to the
+// tile: q | dir << 5 | box << 10 | (si & 1) << 18 | (sj & 1) << 19 | (sk & 1) << 20, with dir the
+// 27-neighbour index of the tile holding the source and box the local index of the lowest trilinear
+// corner. Per axis the upper corner weighs t = 1/4 for an odd source coordinate and t = 3/4 for an even
+// one, the lower corner 1 - t.
+constexpr int kFillBox = 6;
 // Slot of (q, destination cell) in a tile's ghost record; valid when the source cell - c_q is outside.
 __host__ __device__ inline int ghost_base(int q) { return q < 7 ? (q - 1) * 64 : 384 + (q - 7) * 120; }
 __host__ __device__ inline int ghost_slot(int q, int i, int j, int k) {

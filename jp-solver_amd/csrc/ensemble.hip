@@ -38,10 +38,10 @@ __global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md
     const MemberDesc& d = md[w.x];
     if constexpr (kLevel == 1)
       fill_l1_tile<S>(cr<S>(d.c0[p]), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.nb1, d.slots, d.G, d.nqs01, d.dec[0], w.y,
-                      threadIdx.x, 256, L);
+                      threadIdx.x, L);
     else
       fill_l2_tile<S>(cr<S>(d.f1[p]), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.nb2, d.slots, d.G,
-                      d.nqs12, d.dec[1], w.y, threadIdx.x, 256, L);
+                      d.nqs12, d.dec[1], w.y, threadIdx.x, L);
   }
 }
 // items: (member, fine tile)

@@ -126,25 +126,34 @@ template <int q, class R> __device__ inline R deq(const EqPre<R>& e, R drho, R r
   }
 }
 
-// Second-order Hermite projection of a non-equilibrium vector (src/tgv3d_amr.cu:56-67).
+// Second-order Hermite projection of a non-equilibrium vector (src/tgv3d_amr.cu:56-67). Terms whose
+// coefficient c_a c_b is zero are dropped at compile time: the accumulators start at +0 and can never
+// become -0, so adding the reference's 0 * v terms leaves them unchanged; of the off-diagonal products
+// each direction has at most one non-zero, and the others only add zeros, whose sign difference is lost
+// as soon as the projected value is added to a (non-zero) equilibrium by every caller.
 template <class R> __device__ inline void regularize(R* n) {
   R Pxx = R(0), Pyy = R(0), Pzz = R(0), Pxy = R(0), Pxz = R(0), Pyz = R(0);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
+    constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
     const R v = n[q];
-    Pxx += R(CX(q) * CX(q)) * v;
-    Pyy += R(CY(q) * CY(q)) * v;
-    Pzz += R(CZ(q) * CZ(q)) * v;
-    Pxy += R(CX(q) * CY(q)) * v;
-    Pxz += R(CX(q) * CZ(q)) * v;
-    Pyz += R(CY(q) * CZ(q)) * v;
+    if constexpr (cx != 0) Pxx = Pxx + v;
+    if constexpr (cy != 0) Pyy = Pyy + v;
+    if constexpr (cz != 0) Pzz = Pzz + v;
+    if constexpr (cx * cy != 0) Pxy = cx * cy > 0 ? Pxy + v : Pxy - v;
+    if constexpr (cx * cz != 0) Pxz = cx * cz > 0 ? Pxz + v : Pxz - v;
+    if constexpr (cy * cz != 0) Pyz = cy * cz > 0 ? Pyz + v : Pyz - v;
   });
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
+    constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
     const R third = R(1) / R(3);
-    const R Qxx = R(CX(q) * CX(q)) - third, Qyy = R(CY(q) * CY(q)) - third, Qzz = R(CZ(q) * CZ(q)) - third;
-    const R Qxy = R(CX(q) * CY(q)), Qxz = R(CX(q) * CZ(q)), Qyz = R(CY(q) * CZ(q));
-    n[q] = R(4.5) * W<R>(q) * (Qxx * Pxx + Qyy * Pyy + Qzz * Pzz + R(2) * (Qxy * Pxy + Qxz * Pxz + Qyz * Pyz));
+    const R Qxx = R(cx * cx) - third, Qyy = R(cy * cy) - third, Qzz = R(cz * cz) - third;
+    R d = Qxx * Pxx + Qyy * Pyy + Qzz * Pzz;
+    if constexpr (cx * cy != 0) d = d + R(2) * (cx * cy > 0 ? Pxy : -Pxy);
+    else if constexpr (cx * cz != 0) d = d + R(2) * (cx * cz > 0 ? Pxz : -Pxz);
+    else if constexpr (cy * cz != 0) d = d + R(2) * (cy * cz > 0 ? Pyz : -Pyz);
+    n[q] = R(4.5) * W<R>(q) * d;
   });
 }
 
