@@ -139,6 +139,7 @@ def main():
     prec = jpabi.PREC_NAMES[args.prec]
     n = args.n
     copy_gbps, _ = jpabi.probe_copy(4 << 30, iters=10)
+    attainable = jpabi.probe_bandwidth(4 << 30, iters=10)
     g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=prec)
     g.init()
     print(f"# uniform {n}^3 {args.prec}: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr,
@@ -171,7 +172,8 @@ def main():
                    "precision": args.prec, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
-        "extra": {"copy_kernel_gbps": copy_gbps, "bytes_per_update": bpu, "pool_gb": r["pool_gb"],
+        "extra": {"copy_kernel_gbps": copy_gbps, "attainable_hbm_gbps": attainable,
+                  "frac_of_attainable_copy": achieved / attainable["copy"], "bytes_per_update": bpu, "pool_gb": r["pool_gb"],
                   "diag_after": [E, ens, mass]},
     }
     if not args.no_extras:

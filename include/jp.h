@@ -219,6 +219,9 @@ void jp_destroy(jp_ctx* c);
 /* Bandwidth probe: copies `bytes` device-to-device with a plain streaming kernel `iters` times and
  * reports GB/s (read + write). Calibrates the roofline and the HBM traffic counters. */
 int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_iter);
+/* Attainable HBM bandwidth of plain kernels (one 16 B element per lane, one pass): mode 0 copy (GB/s
+ * of read + write), 1 read only, 2 write only; bytes rounded down to a multiple of 4096. */
+int jp_probe_bandwidth(int device, int mode, size_t bytes, int iters, double* gbps);
 
 /* Library version string. */
 const char* jp_version(void);

@@ -469,9 +469,11 @@ void jp_destroy(jp_ctx* c) {
   delete c;
 }
 
-int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_iter) {
-  if (!gbps || iters < 1 || bytes < 16) return JP_ERR_ARG;
-  bytes &= ~size_t(15);
+// Times `iters` launches of a plain bandwidth kernel over fresh buffers (mode -1: the streaming copy
+// k_copy16 used to calibrate the traffic counters; 0 copy, 1 read, 2 write).
+static int probe(int device, int mode, size_t bytes, int iters, double* gbps, double* ms_per_iter) {
+  if (!gbps || iters < 1 || bytes < 4096) return JP_ERR_ARG;
+  bytes &= ~size_t(4095);
   if (hipSetDevice(device) != hipSuccess) return JP_ERR_HIP;
   void *a = nullptr, *b = nullptr;
   hipStream_t st;
@@ -482,19 +484,21 @@ int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_
     return JP_ERR_NOMEM;
   }
   int rc = JP_OK;
+  auto launch = [&]() { return mode < 0 ? copy_probe(a, b, bytes, st) : bandwidth_probe(mode, a, b, bytes, st); };
   hipEvent_t e0, e1;
   (void)hipMemsetAsync(a, 0, bytes, st);
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  if (copy_probe(a, b, bytes, st) != hipSuccess) rc = JP_ERR_HIP;
+  if (launch() != hipSuccess) rc = JP_ERR_HIP;
   (void)hipEventRecord(e0, st);
   for (int i = 0; i < iters && rc == JP_OK; ++i)
-    if (copy_probe(a, b, bytes, st) != hipSuccess) rc = JP_ERR_HIP;
+    if (launch() != hipSuccess) rc = JP_ERR_HIP;
   (void)hipEventRecord(e1, st);
   if (hipEventSynchronize(e1) != hipSuccess) rc = JP_ERR_HIP;
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  *gbps = 2.0 * double(bytes) * iters / (double(ms) * 1e6);
+  const double moved = (mode <= 0 ? 2.0 : 1.0) * double(bytes);
+  *gbps = moved * iters / (double(ms) * 1e6);
   if (ms_per_iter) *ms_per_iter = ms / iters;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -502,6 +506,15 @@ int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_
   (void)hipFree(b);
   (void)hipStreamDestroy(st);
   return rc;
+}
+
+int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_per_iter) {
+  return probe(device, -1, bytes, iters, gbps, ms_per_iter);
+}
+
+int jp_probe_bandwidth(int device, int mode, size_t bytes, int iters, double* gbps) {
+  if (mode < 0 || mode > 2) return JP_ERR_ARG;
+  return probe(device, mode, bytes, iters, gbps, nullptr);
 }
 
 }  // extern "C"

@@ -42,6 +42,8 @@ hipError_t field_diag(bool dbl_field, bool diff_in_field_type, const void* field
                       int nblocks, double* out3, hipStream_t st);
 constexpr int kDiagBlocks = 1024;
 hipError_t copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
+// mode 0 copy src -> dst, 1 read src, 2 write dst; bytes a multiple of 4096
+hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st);
 // tile-major <-> dense plane conversion for the test hooks (raw elements, any precision)
 hipError_t dense_to_tiles(int prec, const void* f, const Dense& g, void* tiles, hipStream_t st);
 hipError_t tiles_to_dense(int prec, const void* tiles, const Dense& g, void* f, hipStream_t st);

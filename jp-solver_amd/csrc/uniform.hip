@@ -139,6 +139,21 @@ __global__ void k_copy16(const u32x4* __restrict__ s, u32x4* __restrict__ d, siz
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
     __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
 }
+// Plain bandwidth kernels, one 16 B element per lane and one pass over the buffer (the fastest simple
+// forms on MI355X): mode 0 copies s to d, 1 reads s (the sum is stored only if it matches a value the
+// zero-filled buffer cannot produce, so the loads stay live), 2 writes d.
+template <int kMode> __global__ __launch_bounds__(256) void k_probe(const u32x4* __restrict__ s, u32x4* __restrict__ d) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if constexpr (kMode == 0) {
+    d[i] = s[i];
+  } else if constexpr (kMode == 1) {
+    const u32x4 v = s[i];
+    if ((v.x ^ v.y ^ v.z ^ v.w) == 0x9e3779b9u) d[0] = v;
+  } else {
+    const u32x4 v = {uint32_t(i), 1u, 2u, 3u};
+    d[i] = v;
+  }
+}
 
 template <class T, bool kToTiles>
 __global__ void k_dense_tiles(const T* __restrict__ src, T* __restrict__ dst, Dense g) {
@@ -281,6 +296,17 @@ hipError_t field_diag(bool dbl, bool diff_in_field, const void* field, int N, do
   }
   const double inv = 1.0 / (double(N) * N * N);
   hipLaunchKernelGGL((k_diag_final<kBS>), dim3(1), dim3(kBS), 0, st, partial, nblocks, inv, out3);
+  return hipGetLastError();
+}
+
+hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;  // a multiple of 256
+  const dim3 grid(uint32_t(n / 256));
+  auto s = static_cast<const u32x4*>(src);
+  auto d = static_cast<u32x4*>(dst);
+  if (mode == 0) hipLaunchKernelGGL(k_probe<0>, grid, dim3(256), 0, st, s, d);
+  else if (mode == 1) hipLaunchKernelGGL(k_probe<1>, grid, dim3(256), 0, st, s, d);
+  else hipLaunchKernelGGL(k_probe<2>, grid, dim3(256), 0, st, s, d);
   return hipGetLastError();
 }
 

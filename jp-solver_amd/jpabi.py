@@ -98,6 +98,7 @@ def lib():
             "jp_population_bytes": (i, [vp, i, ctypes.POINTER(sz)]),
             "jp_sync": (i, [vp]), "jp_last_error": (ctypes.c_char_p, [vp]), "jp_destroy": (None, [vp]),
             "jp_probe_copy": (i, [i, sz, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
+            "jp_probe_bandwidth": (i, [i, i, sz, i, ctypes.POINTER(d)]),
             "jp_amr_create": (i, [ctypes.POINTER(AmrParams), ctypes.POINTER(vp)]),
             "jp_regrid": (i, [vp]), "jp_init_with_sensor": (i, [vp, vp, vp]),
             "jp_regrid_with_sensor": (i, [vp, vp, vp]),
@@ -120,6 +121,18 @@ def declared_symbols(header=None):
     header = header or os.path.join(HERE, "..", "include", "jp.h")
     txt = open(header).read()
     return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+\**(jp_\w+)\s*\(", txt, re.M)))
+
+
+def probe_bandwidth(nbytes, iters=10, device=0):
+    """Plain-kernel HBM bandwidth in GB/s: {"copy": read + write, "read": ..., "write": ...}."""
+    out = {}
+    for mode, name in enumerate(("copy", "read", "write")):
+        g = ctypes.c_double()
+        rc = lib().jp_probe_bandwidth(device, mode, nbytes, iters, ctypes.byref(g))
+        if rc != JP_OK:
+            raise JPError(rc, "probe_bandwidth failed")
+        out[name] = g.value
+    return out
 
 
 def probe_copy(nbytes, iters=20, device=0):
