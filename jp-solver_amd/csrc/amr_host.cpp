@@ -318,37 +318,60 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
 }
 
 // ---- one coarse step of the nested 1:2:4 subcycle, src/tgv3d_amr.cu:821-837 ----
+// The level-2 pair of one level-1 substep: ghost fill from level 1, two level-2 substeps, the second
+// restricting into level 1 in its epilogue (src/tgv3d_amr.cu:828-833).
+static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
+  AmrState& a = c->amr;
+  const int p = c->prec;
+  if (!a.skipprep)
+    HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, a.nf2, float(c->d.nqs12),
+                       dec(c, 1), st));
+  HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+  std::swap(a.f2a, a.f2b);
+  if (a.skiprestr)
+    HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+  else
+    HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a,
+                                  a.l2pl1, a.l2oct, float(c->d.nqi21), enc(c, 1), st));
+  std::swap(a.f2a, a.f2b);
+  return JP_OK;
+}
+
+// One coarse step (src/tgv3d_amr.cu:821-837). The first level-2 pair reads and writes only levels 1 and
+// 2 (its restriction touches the interior of refined level-1 tiles), while the level-0 step and the
+// level-1 fill touch only level 0 and the level-1 ghost record: the pair runs on a second stream,
+// overlapping them, and the first level-1 substep waits for both. Every buffer is written by at most one
+// of the two branches and read by the other only after the join, so the results are those of the
+// sequential order.
 int amr_base_step(jp_ctx* c, hipStream_t st) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   const int p = c->prec;
+  const bool fork = a.nf2 > 0;
+  hipEvent_t ev_fork = a.ev_fork[a.evi], ev_join = a.ev_join[a.evi];
+  a.evi ^= 1;  // two event pairs: consecutive steps in one captured graph do not reuse an event
+  if (fork) {
+    HCK(c, hipEventRecord(ev_fork, st));
+    HCK(c, hipStreamWaitEvent(a.st2, ev_fork, 0));
+    if (int r = level2_pair(c, G, a.st2)) return r;
+    HCK(c, hipEventRecord(ev_join, a.st2));
+  }
   HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
   std::swap(c->c0a, c->c0b);
   if (!a.nf1) return JP_OK;
   if (!a.skipprep)  // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
     HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, a.nf1, float(c->d.nqs01), dec(c, 0), st));
-  for (int s1 = 0; s1 < 2; ++s1) {
-    if (a.nf2) {
-      if (!a.skipprep)
-        HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, a.nf2,
-                           float(c->d.nqs12), dec(c, 1), st));
-      // the second level-2 substep carries the restriction into level 1 in its epilogue
-      HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
-      std::swap(a.f2a, a.f2b);
-      if (a.skiprestr)
-        HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
-      else
-        HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2),
-                                      a.f1a, a.l2pl1, a.l2oct, float(c->d.nqi21), enc(c, 1), st));
-      std::swap(a.f2a, a.f2b);
-    }
-    if (s1 == 1 && !a.skiprestr)  // the second level-1 substep restricts into level 0
-      HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
-                                    a.l1pcb, a.l1oct, float(c->d.nqi10), enc(c, 0), st));
-    else
-      HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
-    std::swap(a.f1a, a.f1b);
-  }
+  if (fork) HCK(c, hipStreamWaitEvent(st, ev_join, 0));
+  HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+  std::swap(a.f1a, a.f1b);
+  if (a.nf2)
+    if (int r = level2_pair(c, G, st)) return r;
+  if (a.skiprestr)
+    HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+  else  // the second level-1 substep restricts into level 0
+    HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
+                                  a.l1pcb, a.l1oct, float(c->d.nqi10), enc(c, 0), st));
+  std::swap(a.f1a, a.f1b);
   return JP_OK;
 }
 
@@ -407,6 +430,11 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
   while (a.cap_nref0 > 1 && !pool_fits(a, 8L * a.cap_nref0, 0)) a.cap_nref0--;
   a.cap1 = 8 * a.cap_nref0;
   a.cap_nref1 = INT_MAX;
+  HCK(c, hipStreamCreate(&a.st2));  // default priority, like every stream of the library
+  for (int i = 0; i < 2; ++i) {
+    HCK(c, hipEventCreateWithFlags(&a.ev_fork[i], hipEventDisableTiming));
+    HCK(c, hipEventCreateWithFlags(&a.ev_join[i], hipEventDisableTiming));
+  }
   // pools
   c->g0 = make_dense(N, 0);
   c->variant = default_variant(c->prec);

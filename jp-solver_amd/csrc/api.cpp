@@ -465,6 +465,14 @@ void jp_destroy(jp_ctx* c) {
   ens_destroy(c);
   for (auto& a : c->allocs) (void)hipFree(a.first);
   if (c->amr.hcnt) (void)hipHostFree(c->amr.hcnt);
+  if (c->amr.st2) {
+    (void)hipStreamSynchronize(c->amr.st2);
+    (void)hipStreamDestroy(c->amr.st2);
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (c->amr.ev_fork[i]) (void)hipEventDestroy(c->amr.ev_fork[i]);
+    if (c->amr.ev_join[i]) (void)hipEventDestroy(c->amr.ev_join[i]);
+  }
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

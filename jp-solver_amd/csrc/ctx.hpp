@@ -23,7 +23,10 @@ struct AmrState {
   int nref0 = 0, nf1 = 0, nref1 = 0, nf2 = 0, n_skin0 = 0, n_skin1 = 0;
   bool f1_stale = false;
   int cap2_alloc = 0;  // level-2 tiles the allocated pool holds
-  int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)  // f1b still has a pre-growth size (until the regrid's migration completes)
+  int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)
+  hipStream_t st2 = nullptr;  // side stream of the coarse step (first level-2 pair)
+  hipEvent_t ev_fork[2] = {}, ev_join[2] = {};
+  int evi = 0;  // f1b still has a pre-growth size (until the regrid's migration completes)
   void *f1a = nullptr, *f1b = nullptr, *f2a = nullptr, *f2b = nullptr;  // stored populations
   void *gh1 = nullptr, *gh2 = nullptr;  // ghost records, working-type values
   int *ref0 = nullptr, *fof0 = nullptr, *rl0 = nullptr, *l1pcb = nullptr, *l1oct = nullptr, *f1pos = nullptr;

@@ -22,14 +22,27 @@ __device__ inline float fkey_inv(uint32_t k) { return __uint_as_float((k & 0x800
 constexpr int kSelBS = 1024;
 
 // One workgroup. count = #entries (mask != 0); tgt = min(cap, max(1, int(frac*count)));
-// thr = value at ascending rank count - tgt.
+// thr = value at ascending rank count - tgt. Four 8-bit radix passes over the orderable keys; the keys
+// and their mask bits stay in LDS between passes when they fit. The digit holding the rank is found by
+// one wave: each lane owns 4 bins, a wave prefix sum gives the
+// cumulative counts and the lane whose range contains the rank resolves it (the first bin whose
+// cumulative count exceeds the rank, as a sequential walk would).
+constexpr int kSelLds = 12288;
 __global__ __launch_bounds__(kSelBS) void k_select(const float* __restrict__ sen, const int* __restrict__ mask, int n,
                                                    double frac, int cap, float* __restrict__ thr, int* __restrict__ cnt,
                                                    int count_slot) {
   __shared__ unsigned hist[256];
   __shared__ unsigned s_prefix, s_rank;
   __shared__ int s_count;
+  __shared__ uint32_t keys[kSelLds];
+  __shared__ uint8_t valid[kSelLds];
   const int tid = threadIdx.x;
+  const bool in_lds = n <= kSelLds;
+  if (in_lds)
+    for (int i = tid; i < n; i += kSelBS) {
+      valid[i] = (!mask || mask[i]) ? 1 : 0;
+      keys[i] = fkey(sen[i]);
+    }
   unsigned prefix = 0;
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = 24 - 8 * pass;
@@ -37,31 +50,59 @@ __global__ __launch_bounds__(kSelBS) void k_select(const float* __restrict__ sen
     for (int b = tid; b < 256; b += kSelBS) hist[b] = 0;
     __syncthreads();
     for (int i = tid; i < n; i += kSelBS) {
-      if (mask && !mask[i]) continue;
-      const uint32_t k = fkey(sen[i]);
+      uint32_t k;
+      if (in_lds) {
+        if (!valid[i]) continue;
+        k = keys[i];
+      } else {
+        if (mask && !mask[i]) continue;
+        k = fkey(sen[i]);
+      }
       if ((k & himask) == (prefix & himask)) atomicAdd(&hist[(k >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
+      const unsigned h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+      const unsigned local = h0 + h1 + h2 + h3;
+      unsigned incl = local;
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = __shfl_up(incl, off, 64);
+        if (tid >= off) incl += v;
+      }
+      const unsigned tot = __shfl(incl, 63, 64);
+      unsigned r;
       if (pass == 0) {
-        unsigned tot = 0;
-        for (int b = 0; b < 256; ++b) tot += hist[b];
-        s_count = int(tot);
         int tgt = int(frac * double(tot));
         tgt = tgt < 1 ? 1 : tgt;
         tgt = tgt > cap ? cap : tgt;
-        s_rank = tot > 0 ? unsigned(int(tot) - tgt) : 0u;
-        cnt[count_slot] = int(tot);
-        cnt[CNT_TGT] = tgt;
+        r = tot > 0 ? unsigned(int(tot) - tgt) : 0u;
+        if (tid == 0) {
+          s_count = int(tot);
+          cnt[count_slot] = int(tot);
+          cnt[CNT_TGT] = tgt;
+        }
+      } else {
+        r = s_rank;
       }
-      unsigned r = s_rank, acc = 0;
-      int d = 0;
-      for (; d < 255; ++d) {
-        if (acc + hist[d] > r) break;
-        acc += hist[d];
+      const unsigned excl = incl - local;
+      if (excl <= r && r < incl) {  // exactly one lane when r < tot
+        unsigned acc = excl;
+        int d = 4 * tid;
+        if (acc + h0 <= r) {
+          acc += h0;
+          ++d;
+          if (acc + h1 <= r) {
+            acc += h1;
+            ++d;
+            if (acc + h2 <= r) {
+              acc += h2;
+              ++d;
+            }
+          }
+        }
+        s_rank = r - acc;
+        s_prefix = prefix | (unsigned(d) << shift);
       }
-      s_rank = r - acc;
-      s_prefix = prefix | (unsigned(d) << shift);
     }
     __syncthreads();
     prefix = s_prefix;
