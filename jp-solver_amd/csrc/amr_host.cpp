@@ -88,20 +88,29 @@ static void free_l2_pool(jp_ctx* c) {
   }
 }
 
-// skins (src/tgv3d_amr.cu:758-762) and the 27-neighbour tables of both fine levels
+// The 27-neighbour tables of both fine levels, which the steps and fills read. The skin lists
+// (src/tgv3d_amr.cu:758-762) are part of the block tables but no kernel here reads them (the fills compute
+// prep where they need it), so they are built on demand by ensure_skins.
 static int finish_tables(jp_ctx* c) {
   AmrState& a = c->amr;
+  if (a.nf1) HCK(c, rg_nbr_l1(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.nb1, c->st));
+  if (a.nf2) HCK(c, rg_nbr_l2(a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.fof1, a.l1gx, a.nf2, a.nb2, c->st));
+  a.skins_stale = true;
+  return JP_OK;
+}
+static int ensure_skins(jp_ctx* c) {
+  AmrState& a = c->amr;
+  if (!a.skins_stale) return JP_OK;
   HCK(c, rg_dilate_l0(a.ref0, a.cnb0, a.cgx, a.flags, c->st));
   HCK(c, rg_compact(a.flags, a.cnb0, a.skin0, a.dcnt, CNT_SKIN0, c->st));
   if (a.nf1) {
     HCK(c, rg_dilate_l1(a.ref1, a.f1pos, a.l1grid, a.nf1, a.l1gx, a.flags, c->st));
     HCK(c, rg_compact(a.flags, a.nf1, a.skin1, a.dcnt, CNT_SKIN1, c->st));
-    HCK(c, rg_nbr_l1(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.nb1, c->st));
   }
-  if (a.nf2) HCK(c, rg_nbr_l2(a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.fof1, a.l1gx, a.nf2, a.nb2, c->st));
   if (int r = read_counts(c)) return r;
   a.n_skin0 = a.hcnt[CNT_SKIN0];
   a.n_skin1 = a.nf1 ? a.hcnt[CNT_SKIN1] : 0;
+  a.skins_stale = false;
   return JP_OK;
 }
 
@@ -249,11 +258,12 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   drop_graph(c);
-  HCK(c, rg_copy_i32(a.ref0, a.o_ref0, size_t(a.cnb0), c->st));
-  HCK(c, rg_copy_i32(a.fof0, a.o_fof0, size_t(a.cnb0), c->st));
-  HCK(c, rg_copy_i32(a.l1grid, a.o_l1grid, size_t(a.NG1), c->st));
-  HCK(c, rg_copy_i32(a.ref1, a.o_ref1, size_t(a.cap1), c->st));
-  HCK(c, rg_copy_i32(a.fof1, a.o_fof1, size_t(a.cap1), c->st));
+  {  // snapshots of the tables the migration keys on, one launch
+    const int* src[5] = {a.ref0, a.fof0, a.l1grid, a.ref1, a.fof1};
+    int* dst[5] = {a.o_ref0, a.o_fof0, a.o_l1grid, a.o_ref1, a.o_fof1};
+    const long n[5] = {a.cnb0, a.cnb0, a.NG1, a.cap1, a.cap1};
+    HCK(c, rg_copy_i32_multi(src, dst, n, 5, c->st));
+  }
   HCK(c, amr_sensor(c->prec, 0, G, c->c0a, a.cnb0, dec(c, 0), a.sen0, c->st));
   if (sen0)
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
@@ -477,6 +487,7 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
 }
 
 int amr_counts(jp_ctx* c, jp_counts* o) {
+  if (int r = ensure_skins(c)) return r;
   const AmrState& a = c->amr;
   o->cnb0 = a.cnb0;
   o->nref0 = a.nref0;
@@ -496,6 +507,8 @@ int amr_counts(jp_ctx* c, jp_counts* o) {
 }
 
 int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n) {
+  if (which == JP_TABLE_SKIN0 || which == JP_TABLE_SKIN1)
+    if (int r = ensure_skins(c)) return r;
   const AmrState& a = c->amr;
   const void* src = nullptr;
   long len = 0;

@@ -107,6 +107,7 @@ hipError_t rg_nbr_l1(const int* f1pos, const int* l1grid, int l1gx, int nf1, int
 hipError_t rg_nbr_l2(const int* l2pl1, const int* l2oct, const int* f1pos, const int* l1grid, const int* ref1,
                      const int* fof1, int l1gx, int nf2, int* nb2, hipStream_t st);
 hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, float* out, hipStream_t st);
-hipError_t rg_copy_i32(const int* src, int* dst, size_t n, hipStream_t st);
+// up to 5 copies src[k] -> dst[k] of n[k] ints in one launch
+hipError_t rg_copy_i32_multi(const int* const* src, int* const* dst, const long* n, int count, hipStream_t st);
 
 }  // namespace jp

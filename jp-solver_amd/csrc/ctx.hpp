@@ -22,6 +22,7 @@ struct AmrState {
   long hw_nf1 = 0, hw_nf2 = 0;
   int nref0 = 0, nf1 = 0, nref1 = 0, nf2 = 0, n_skin0 = 0, n_skin1 = 0;
   bool f1_stale = false;
+  bool skins_stale = true;  // skin lists and their counts are rebuilt on demand after a regrid
   int cap2_alloc = 0;  // level-2 tiles the allocated pool holds
   int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)
   hipStream_t st2 = nullptr;  // side stream of the coarse step (first level-2 pair)

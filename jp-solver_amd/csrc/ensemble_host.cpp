@@ -24,7 +24,7 @@ struct EnsState {
   int n[EK_COUNT] = {};
   int groups[EK_COUNT] = {};
   int p0 = 0, p1 = 0, p2 = 0;  // parity of the current buffer of levels 0, 1, 2
-  std::vector<long> sig;       // member counts at upload (nf1, nf2, skins)
+  std::vector<long> sig;       // member counts and pool generation at upload
   bool uploaded = false;
   long steps = 0;
 };
@@ -82,8 +82,7 @@ static std::vector<long> signature(const EnsState& e) {
   std::vector<long> s;
   for (const jp_ctx* m : e.m) {
     const AmrState& a = m->amr;
-    s.insert(s.end(), {long(a.nf1), long(a.nf2), long(a.n_skin0), long(a.n_skin1), long(a.nref0), long(a.nref1),
-                       long(a.pool_gen)});
+    s.insert(s.end(), {long(a.nf1), long(a.nf2), long(a.nref0), long(a.nref1), long(a.pool_gen)});
   }
   return s;
 }

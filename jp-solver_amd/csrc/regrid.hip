@@ -7,6 +7,7 @@
 // copied to the host. The scan and the compaction use wavefront ballots: a 0/1 flag per lane, the
 // prefix within the wave from mbcnt, wave totals combined through LDS. All results are exact and
 // independent of scheduling.
+#include <algorithm>
 #include <climits>
 
 #include "amr_internal.hpp"
@@ -294,9 +295,6 @@ __global__ void k_gather_by_pos(const float* __restrict__ by_pos, const int* __r
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b < nf1) out[b] = by_pos[f1pos[b]];
 }
-__global__ void k_copy_i32(const int* __restrict__ s, int* __restrict__ d, size_t n) {
-  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) d[i] = s[i];
-}
 
 static inline dim3 g256(long n) { return dim3(unsigned((n + 255) / 256)); }
 
@@ -373,9 +371,31 @@ hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, floa
   hipLaunchKernelGGL(k_gather_by_pos, g256(nf1), dim3(256), 0, st, by_pos, f1pos, nf1, out);
   return hipGetLastError();
 }
-hipError_t rg_copy_i32(const int* src, int* dst, size_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_i32, dim3(unsigned(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, st, src, dst, n);
+struct CopyList {
+  const int* src[5];
+  int* dst[5];
+  long n[5];
+  int count;
+};
+__global__ void k_copy_i32_multi(CopyList L, long nmax) {
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= nmax) return;
+  for (int k = 0; k < L.count; ++k)
+    if (i < L.n[k]) L.dst[k][i] = L.src[k][i];
+}
+hipError_t rg_copy_i32_multi(const int* const* src, int* const* dst, const long* n, int count, hipStream_t st) {
+  if (count < 1 || count > 5) return hipErrorInvalidValue;
+  CopyList L{};
+  long nmax = 0;
+  for (int k = 0; k < count; ++k) {
+    L.src[k] = src[k];
+    L.dst[k] = dst[k];
+    L.n[k] = n[k];
+    nmax = std::max(nmax, n[k]);
+  }
+  L.count = count;
+  if (nmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_i32_multi, dim3(unsigned((nmax + 255) / 256)), dim3(256), 0, st, L, nmax);
   return hipGetLastError();
 }
 
