@@ -29,12 +29,13 @@ __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __re
                                                      const int* __restrict__ ptile, const int* __restrict__ poct, AmrGeo G,
                                                      float nqi, float cenc) {
   __shared__ int snb[kNbr];
-  __shared__ typename S::real keep[Q * MB3];
+  __shared__ RestrictLds<typename S::real> L;
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
   __syncthreads();
-  fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x, keep);
-  restrict_tile_lds<S, kToDense>(keep, threadIdx.x, cf, ptile[b], poct[b], G, nqi, cenc);
+  typename S::real kept[Q];
+  fine_step_cell<S, true>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x, kept);
+  restrict_tile<S, kToDense>(kept, threadIdx.x, L, cf, ptile[b], poct[b], G, nqi, cenc);
 }
 
 // Ghost fills, one fine tile per workgroup (the parent box and prep values staged in LDS).

@@ -57,51 +57,37 @@ template <class S> __device__ inline void load_dense(const typename S::raw* __re
 }
 
 // BGK collision of pulled values (the form's working representation), stored to tile b
-// (src/tgv3d_amr.cu:268-273). keep (optional, LDS [19][512]): receives each stored population as it reads
+// (src/tgv3d_amr.cu:268-273). kept (optional, 19 registers): receives each stored population as it reads
 // back, for the fused restriction.
-template <class S> __device__ inline void collide_store(typename S::real* fin, typename S::real om,
-                                                        typename S::raw* __restrict__ fb, int b, int cell, float enc,
-                                                        typename S::real* keep = nullptr, float dec = 1.f) {
+template <class S, bool kKeep = false>
+__device__ inline void collide_store(typename S::real (&fin)[Q], typename S::real om, typename S::raw* __restrict__ fb,
+                                     int b, int cell, float enc, typename S::real (&kept)[Q], float dec) {
   const Collider<S> col(fin);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], om), enc);
     fb[tidx(b, q, cell)] = v;
-    if (keep) keep[q * MB3 + cell] = S::template load<q>(v, dec);
+    if constexpr (kKeep) kept[q] = S::template load<q>(v, dec);
   });
 }
 
-// Restriction of one tile's 64 parent cells from the tile's just-stored populations held in LDS
-// (src/tgv3d_amr.cu:275-298, 358-381): parent = feq + nqi * regularize(mean of the 8 children - feq).
-// The 8 children of a parent cell lie in this tile, so the average needs no second pass over the fine
-// level. The 512 lanes first sum (in the reference's child order) 64 parents x 19 directions, then 64
-// lanes finish one parent cell each. Call with all lanes.
+// Restriction of one tile's 64 parent cells from the tile's just-stored populations, fused into the fine
+// step (src/tgv3d_amr.cu:275-298, 358-381): parent = feq + nqi * regularize(mean of the 8 children - feq).
+// The 8 children of a parent cell lie in this tile, so no second pass over the fine level is needed. Lane
+// = cell, so wave k holds the z-plane k of the tile and a parent's children are a 2x2 lane patch (lane +
+// 0, 1, 8, 9) in waves 2pk and 2pk + 1. The children are summed in the reference's order c = 0..7 (di, dj,
+// dk): the even wave adds its four with cross-lane reads and leaves the partial sum in LDS, the odd wave
+// adds its four and its 16 patch-corner lanes finish one parent each. Call with all 512 lanes.
+template <class R> struct RestrictLds {
+  R part[HALF][Q][16];  // partial sums per plane pair, direction, parent of the pair's 4x4 layer
+};
+// Parent cell (pi, pj, pk) of a fine tile's octant from its children's mean: feq + nqi * regularize(avg -
+// feq), stored to the parent tile pcb (or the dense level 0).
 template <class S, bool kToDense>
-__device__ inline void restrict_tile_lds(typename S::real* lds, int cell, typename S::raw* __restrict__ cf, int pcb,
-                                         int oct, const AmrGeo& G, float nqi, float cenc) {
+__device__ inline void restrict_finish(typename S::real (&avg)[Q], typename S::raw* __restrict__ cf, int pcb, int oct, int pi,
+                                       int pj, int pk, const AmrGeo& G, float nqi, float cenc) {
   using R = typename S::real;
-  __syncthreads();
-  const int p = cell & 63, part = cell >> 6;  // parent (pi, pj, pk) = (p & 3, (p >> 2) & 3, p >> 4)
-  const int c0 = ((2 * (p >> 4)) * MB + 2 * ((p >> 2) & 3)) * MB + 2 * (p & 3);
-  R s[3];
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int q = part + 8 * u;
-    R acc = R(0);
-    if (q < Q) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) acc += R(0.125) * lds[q * MB3 + c0 + (c >> 2) * 64 + ((c >> 1) & 1) * 8 + (c & 1)];
-    }
-    s[u] = acc;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (part + 8 * u < Q) lds[(part + 8 * u) * 64 + p] = s[u];
-  __syncthreads();
-  if (cell >= 64) return;
-  R avg[Q], nq[Q];
-  sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = lds[decltype(qc)::value * 64 + p]; });
+  R nq[Q];
   const Mom<R> m = moments(avg);
   const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
   const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
@@ -110,7 +96,7 @@ __device__ inline void restrict_tile_lds(typename S::real* lds, int cell, typena
     nq[q] = avg[q] - feq<q>(e);
   });
   regularize(nq);
-  const int lci = 4 * (oct & 1) + (p & 3), lcj = 4 * ((oct >> 1) & 1) + ((p >> 2) & 3), lck = 4 * (oct >> 2) + (p >> 4);
+  const int lci = 4 * (oct & 1) + pi, lcj = 4 * ((oct >> 1) & 1) + pj, lck = 4 * (oct >> 2) + pk;
   const int pc = (lck * MB + lcj) * MB + lci;
   const uint32_t di = kToDense ? dense_of_tile(G, pcb, pc) : 0;
   sfor<Q>([&](auto qc) {
@@ -120,15 +106,43 @@ __device__ inline void restrict_tile_lds(typename S::real* lds, int cell, typena
     else cf[tidx(pcb, q, pc)] = v;
   });
 }
+template <class S, bool kToDense>
+__device__ inline void restrict_tile(const typename S::real (&kept)[Q], int cell, RestrictLds<typename S::real>& L,
+                                     typename S::raw* __restrict__ cf, int pcb, int oct, const AmrGeo& G, float nqi,
+                                     float cenc) {
+  using R = typename S::real;
+  const int k = cell >> 6, lane = cell & 63, i = lane & 7, j = lane >> 3;  // k is uniform in a wave
+  const bool corner = !(i & 1) && !(j & 1);
+  const int pp = (j >> 1) * 4 + (i >> 1);
+  if (!(k & 1)) {  // children c = 0..3 of plane 2pk
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R t = R(0.125) * kept[q];
+      const R t1 = __shfl_down(t, 1, 64), t8 = __shfl_down(t, 8, 64), t9 = __shfl_down(t, 9, 64);
+      if (corner) L.part[k >> 1][q][pp] = ((((R(0) + t) + t1) + t8) + t9);
+    });
+  }
+  __syncthreads();
+  if (k & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum
+    R avg[Q];
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R t = R(0.125) * kept[q];
+      const R t1 = __shfl_down(t, 1, 64), t8 = __shfl_down(t, 8, 64), t9 = __shfl_down(t, 9, 64);
+      avg[q] = corner ? ((((L.part[k >> 1][q][pp] + t) + t1) + t8) + t9) : R(0);
+    });
+    if (corner) restrict_finish<S, kToDense>(avg, cf, pcb, oct, i >> 1, j >> 1, k >> 1, G, nqi, cenc);
+  }
+}
 
 // Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
 // 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
 // to this level); missing sources come from the ghost record.
 // snb: the tile's neighbour entries, staged in LDS by the caller.
-template <class S>
+template <class S, bool kKeep = false>
 __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
-                                      float enc, float dec, int b, int cell, typename S::real* keep = nullptr) {
+                                      float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
   using R = typename S::real;
   const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
   const R* __restrict__ g = gh + size_t(b) * kGhostSlots;
@@ -148,7 +162,16 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
       else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
     }
   });
-  collide_store<S>(fin, om, fb, b, cell, enc, keep, dec);
+  collide_store<S, kKeep>(fin, om, fb, b, cell, enc, kept, dec);
+}
+
+// The same without keeping the stored values.
+template <class S>
+__device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                      const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
+                                      float enc, float dec, int b, int cell) {
+  typename S::real none[Q];
+  fine_step_cell<S, false>(fa, fb, gh, snb, om, enc, dec, b, cell, none);
 }
 
 // prep = feq + nqs * regularize(f - feq) of one cell, in place (src/tgv3d_amr.cu:160-171).

@@ -72,22 +72,23 @@ __global__ __launch_bounds__(512) void kb_fine_step_r(const MemberDesc* __restri
                                                       int n, int pf, int pc) {
   using R = typename S::real;
   __shared__ int snb[kNbr];
-  __shared__ R keep[Q * MB3];
+  __shared__ RestrictLds<R> L;
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
     const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
-    __syncthreads();  // the previous item's lanes are done with snb and keep
+    __syncthreads();  // the previous item's lanes are done with snb and the partial sums
     if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(w.y) * kNbr + threadIdx.x];
     __syncthreads();
+    R kept[Q];
     if constexpr (kLevel == 1) {
-      fine_step_cell<S>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]), d.enc[1],
-                        d.dec[1], w.y, threadIdx.x, keep);
-      restrict_tile_lds<S, true>(keep, threadIdx.x, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
+      fine_step_cell<S, true>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]),
+                              d.enc[1], d.dec[1], w.y, threadIdx.x, kept);
+      restrict_tile<S, true>(kept, threadIdx.x, L, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
     } else {
-      fine_step_cell<S>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]), d.enc[2],
-                        d.dec[2], w.y, threadIdx.x, keep);
-      restrict_tile_lds<S, false>(keep, threadIdx.x, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
+      fine_step_cell<S, true>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]),
+                              d.enc[2], d.dec[2], w.y, threadIdx.x, kept);
+      restrict_tile<S, false>(kept, threadIdx.x, L, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
     }
   }
 }
