@@ -77,9 +77,11 @@ __device__ inline void collide_store(typename S::real (&fin)[Q], typename S::rea
 // = cell, so wave k holds the z-plane k of the tile and a parent's children are a 2x2 lane patch (lane +
 // 0, 1, 8, 9) in waves 2pk and 2pk + 1. The children are summed in the reference's order c = 0..7 (di, dj,
 // dk): the even wave adds its four with cross-lane reads and leaves the partial sum in LDS, the odd wave
-// adds its four and its 16 patch-corner lanes finish one parent each. Call with all 512 lanes.
+// adds its four and leaves the mean in LDS, and one full wave finishes the 64 parents (a lane each).
+// Call with all 512 lanes.
 template <class R> struct RestrictLds {
   R part[HALF][Q][16];  // partial sums per plane pair, direction, parent of the pair's 4x4 layer
+  R avg[Q][64];         // complete child means per direction and parent (pk * 16 + pj * 4 + pi)
 };
 // Parent cell (pi, pj, pk) of a fine tile's octant from its children's mean: feq + nqi * regularize(avg -
 // feq), stored to the parent tile pcb (or the dense level 0).
@@ -124,14 +126,18 @@ __device__ inline void restrict_tile(const typename S::real (&kept)[Q], int cell
   }
   __syncthreads();
   if (k & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum
-    R avg[Q];
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       const R t = R(0.125) * kept[q];
       const R t1 = __shfl_down(t, 1, 64), t8 = __shfl_down(t, 8, 64), t9 = __shfl_down(t, 9, 64);
-      avg[q] = corner ? ((((L.part[k >> 1][q][pp] + t) + t1) + t8) + t9) : R(0);
+      if (corner) L.avg[q][(k >> 1) * 16 + pp] = (((L.part[k >> 1][q][pp] + t) + t1) + t8) + t9;
     });
-    if (corner) restrict_finish<S, kToDense>(avg, cf, pcb, oct, i >> 1, j >> 1, k >> 1, G, nqi, cenc);
+  }
+  __syncthreads();
+  if (k == 0) {  // one full wave finishes the 64 parents, lane = pk * 16 + pj * 4 + pi
+    R avg[Q];
+    sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = L.avg[decltype(qc)::value][lane]; });
+    restrict_finish<S, kToDense>(avg, cf, pcb, oct, lane & 3, (lane >> 2) & 3, lane >> 4, G, nqi, cenc);
   }
 }
 
