@@ -146,26 +146,48 @@ static int grow_l1(jp_ctx* c, bool keep_f1a) {
 }
 
 // level-1 marks -> scan -> counts -> tables (common to init and regrid). ref0 is already marked.
+static int scatter_l1(jp_ctx* c) {
+  AmrState& a = c->amr;
+  if (int r = fill(c, a.l1grid, -1, size_t(a.NG1))) return r;  // the scatter writes every fof0 entry
+  HCK(c, rg_scatter_l1(a.ref0, a.rank, a.cnb0, a.cgx, a.l1gx, a.cap1, a.fof0, a.rl0, a.l1pcb, a.l1oct, a.l1grid, a.f1pos,
+                       a.dcnt, c->st));
+  return JP_OK;
+}
+
+// The level-1 tables are scattered before the host reads the count (the scatter needs only the ranks and
+// refuses writes past the pool), so the launches after the read are not serialised behind it; in the rare
+// case that the count exceeds the pool, the pool grows and the scatter is repeated.
 static int build_l1_tables(jp_ctx* c, bool keep_f1a) {
   AmrState& a = c->amr;
   HCK(c, rg_scan(a.ref0, a.cnb0, a.rank, a.dcnt, CNT_NREF0, c->st));
+  if (int r = scatter_l1(c)) return r;
   if (int r = read_counts(c)) return r;
   a.nref0 = a.hcnt[CNT_NREF0];
   a.nf1 = 8 * a.nref0;
   a.hw_nf1 = std::max(a.hw_nf1, long(a.nf1));
-  if (a.nf1 > a.cap1)
+  if (a.nf1 > a.cap1) {
     if (int r = grow_l1(c, keep_f1a)) return r;
-  if (int r = fill(c, a.l1grid, -1, size_t(a.NG1))) return r;
-  if (int r = fill(c, a.fof0, -1, size_t(a.cnb0))) return r;
-  if (a.nref0)
-    HCK(c, rg_scatter_l1(a.ref0, a.rank, a.cnb0, a.cgx, a.l1gx, a.cap1, a.fof0, a.rl0, a.l1pcb, a.l1oct, a.l1grid, a.f1pos,
-                         a.dcnt, c->st));
+    if (int r = fill(c, a.dcnt + CNT_OVERFLOW, 0, 1)) return r;
+    a.hcnt[CNT_OVERFLOW] = 0;
+    if (int r = scatter_l1(c)) return r;
+  }
   return JP_OK;
 }
 
-static int scan_l2(jp_ctx* c) {
+static int scatter_l2(jp_ctx* c) {
+  AmrState& a = c->amr;
+  if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
+  HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
+  return JP_OK;
+}
+
+// Level-1 marks are in ref1: rank them, scatter the level-2 tables ahead of the count read (as above)
+// when the pool exists, and read the level-2 count.
+static int scan_l2(jp_ctx* c, bool scatter) {
   AmrState& a = c->amr;
   HCK(c, rg_scan(a.ref1, a.nf1, a.rank, a.dcnt, CNT_NREF1, c->st));
+  if (scatter)
+    if (int r = scatter_l2(c)) return r;
   if (int r = read_counts(c)) return r;
   a.nref1 = a.hcnt[CNT_NREF1];
   a.nf2 = 8 * a.nref1;
@@ -221,7 +243,7 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, INT_MAX, a.thr, a.dcnt, CNT_NDEEP, c->st));
       HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
     }
-    if (int r = scan_l2(c)) return r;
+    if (int r = scan_l2(c, false)) return r;
     a.cap_nref1 = std::max(1, int(std::ceil(a.nref1 * 2.5)));
     if (a.l2_mode == JP_MODE_ALL) a.cap_nref1 = std::max(a.cap_nref1, a.nf1);
     a.cap_nref1 = std::min(a.cap_nref1, a.cap1);
@@ -269,8 +291,12 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
   HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
   HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
+  HCK(c, rg_differs(a.ref0, a.o_ref0, a.cnb0, a.dcnt, CNT_DIFF0, c->st));
   if (int r = build_l1_tables(c, true)) return r;
-  if (a.nf1) {
+  // Unchanged level-0 marks give the same level-1 tiles in the same order: the migration would be an
+  // identity copy and is skipped (likewise for level 2 below).
+  const bool same1 = !a.hcnt[CNT_DIFF0] && !a.f1_stale;
+  if (a.nf1 && !same1) {
     HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, a.nf1, float(c->d.nqs01),
                           dec(c, 0), enc(c, 1), c->st));
     std::swap(a.f1a, a.f1b);
@@ -288,7 +314,8 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, a.cap_nref1, a.thr, a.dcnt, CNT_NDEEP, c->st));
     if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
     HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
-    if (int r = scan_l2(c)) return r;
+    HCK(c, rg_differs(a.ref1, a.o_ref1, a.nf1, a.dcnt, CNT_DIFF1, c->st));
+    if (int r = scan_l2(c, true)) return r;
     // Ties at the threshold can exceed the level-2 pool (the reference writes out of bounds there): the
     // pool grows when the budget allows; the refinement target cap_nref1 stays as measured at init.
     bool grown = false;
@@ -308,9 +335,12 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       a.f2a = keep;
       grown = true;
     }
-    if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
-    if (a.nref1) HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
-    if (a.nf2) {
+    if (grown) {  // the scatter ahead of the count read stopped at the old pool
+      if (int r = fill(c, a.dcnt + CNT_OVERFLOW, 0, 1)) return r;
+      a.hcnt[CNT_OVERFLOW] = 0;
+      if (int r = scatter_l2(c)) return r;
+    }
+    if (a.nf2 && !(same1 && !a.hcnt[CNT_DIFF1] && !grown)) {
       HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
                             a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
       std::swap(a.f2a, a.f2b);

@@ -82,7 +82,11 @@ hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int n
 
 // ---- regrid.hip ----
 // Device counters (int slots of AmrState::dcnt).
-enum { CNT_NREF0 = 0, CNT_NREF1, CNT_SKIN0, CNT_SKIN1, CNT_NDEEP, CNT_OVERFLOW, CNT_TGT, CNT_NSEL0, CNT_N };
+enum {
+  CNT_NREF0 = 0, CNT_NREF1, CNT_SKIN0, CNT_SKIN1, CNT_NDEEP, CNT_OVERFLOW, CNT_TGT, CNT_NSEL0,
+  CNT_DIFF0, CNT_DIFF1,  // a regrid changed the level-0 / level-1 marks
+  CNT_N
+};
 // thr = ascending-sorted[count - tgt] of sen (entries with mask != 0 when mask is given), tgt =
 // min(cap, max(1, int(frac * count))); writes count to cnt[count_slot] and thr to *thr.
 hipError_t rg_select(const float* sen, const int* mask, int n, double frac, int cap, float* thr, int* cnt,
@@ -107,6 +111,8 @@ hipError_t rg_nbr_l1(const int* f1pos, const int* l1grid, int l1gx, int nf1, int
 hipError_t rg_nbr_l2(const int* l2pl1, const int* l2oct, const int* f1pos, const int* l1grid, const int* ref1,
                      const int* fof1, int l1gx, int nf2, int* nb2, hipStream_t st);
 hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, float* out, hipStream_t st);
+// cnt[slot] = 1 if a[i] != b[i] for some i < n, else 0
+hipError_t rg_differs(const int* a, const int* b, int n, int* cnt, int slot, hipStream_t st);
 // up to 5 copies src[k] -> dst[k] of n[k] ints in one launch
 hipError_t rg_copy_i32_multi(const int* const* src, int* const* dst, const long* n, int count, hipStream_t st);
 

@@ -371,6 +371,16 @@ hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, floa
   hipLaunchKernelGGL(k_gather_by_pos, g256(nf1), dim3(256), 0, st, by_pos, f1pos, nf1, out);
   return hipGetLastError();
 }
+__global__ void k_differs(const int* __restrict__ a, const int* __restrict__ b, int n, int* __restrict__ cnt, int slot) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && a[i] != b[i]) atomicOr(&cnt[slot], 1);
+}
+hipError_t rg_differs(const int* a, const int* b, int n, int* cnt, int slot, hipStream_t st) {
+  if (hipError_t e = fill_i32(cnt + slot, 0, 1, st)) return e;
+  if (n > 0) hipLaunchKernelGGL(k_differs, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, a, b, n, cnt, slot);
+  return hipGetLastError();
+}
+
 struct CopyList {
   const int* src[5];
   int* dst[5];
