@@ -83,6 +83,24 @@ def bench_ensemble(steps, graph=True, tune=True):
     return r
 
 
+def bench_field(n=512, reps=3):
+    """Frame output of the uniform lattice: macroscopic field, Q-criterion, speed and velocity on the device,
+    then the arrays to the host (what one .vti frame needs, without the file write)."""
+    import jpabi
+
+    g = jpabi.Uniform(n, re=1600.0, u0=0.05)
+    g.init()
+    g.step(2)
+    g.field(velocity=True)  # allocates the output buffers
+    t0 = time.time()
+    for _ in range(reps):
+        g.field(velocity=True)
+    dt = (time.time() - t0) / reps
+    g.close()
+    out_bytes = 5 * 4 * n ** 3
+    return {"n": n, "ms_per_frame": dt * 1e3, "bytes_to_host": out_bytes, "host_gbps": out_bytes / dt / 1e9}
+
+
 def extras(quick=False):
     """Secondary workloads of the same metric (reported beside the headline, never as `value`)."""
     out = {}
@@ -95,6 +113,7 @@ def extras(quick=False):
         "amr_gold_64_256": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, 50 if quick else 200,
                                              graph=True),
         "ensemble32": lambda: bench_ensemble(20 if quick else 60),
+        "field_output_512": lambda: bench_field(512, 2 if quick else 3),
     }
     for k, fn in jobs.items():
         t0 = time.time()
@@ -103,7 +122,8 @@ def extras(quick=False):
         except Exception as ex:  # an extra must not hide the headline measurement
             out[k] = {"error": str(ex)}
         out[k]["wall_s"] = time.time() - t0
-        print(f"# extra {k}: {out[k].get('glups', float('nan')):.3f} GLUPS", file=sys.stderr, flush=True)
+        print(f"# extra {k}: {out[k].get('glups', out[k].get('ms_per_frame', float('nan'))):.3f}", file=sys.stderr,
+              flush=True)
     return out
 
 
@@ -133,7 +153,8 @@ def main():
         fn = {"amr-dynamic": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps),
               "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
-              "uniform": lambda: bench_uniform(args.n, args.prec, args.steps)}[args.only]
+              "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),
+              "field": lambda: bench_field(args.n, 3)}[args.only]
         print(json.dumps({"workload": args.only, **fn()}), flush=True)
         return
     prec = jpabi.PREC_NAMES[args.prec]

@@ -223,6 +223,19 @@ int jp_probe_copy(int device, size_t bytes, int iters, double* gbps, double* ms_
  * of read + write), 1 read only, 2 write only; bytes rounded down to a multiple of 4096. */
 int jp_probe_bandwidth(int device, int mode, size_t bytes, int iters, double* gbps);
 
+/* Output field of a frame (src/tgv3d_uniform.cu:96-140; src/tgv3d_amr.cu:551-615): Q-criterion (periodic
+ * central differences, FP64), speed and velocity of the uniform lattice, or of the AMR composite at 4 N0,
+ * computed on the device; only these arrays cross to the host. edge = jp_field_edge(c) (N or 4 N0); Q and
+ * speed hold edge^3 floats, vel 3 edge^3 (or NULL); n3 is the capacity of Q in entries. Index (k*edge+j)*edge+i. */
+long jp_field_edge(jp_ctx* c);
+int jp_field(jp_ctx* c, float* Q, float* speed, float* vel, long n3);
+/* One .vti frame in the reference's format (ImageData, appended raw, UInt64 headers): uniform writes Q, speed
+ * and velocity (src/tgv3d_uniform.cu:110-140), AMR Q and speed of the composite (src/tgv3d_amr.cu:602-615). */
+int jp_write_vti(jp_ctx* c, const char* path);
+/* AMR: the level-1 and level-2 tiles as .vtp line boxes in 4 N0 coordinates, point data "level"
+ * (src/tgv3d_amr.cu:617-641). */
+int jp_write_boxes(jp_ctx* c, const char* path);
+
 /* Library version string. */
 const char* jp_version(void);
 

@@ -316,6 +316,44 @@ int jp_diag(jp_ctx* c, double out[3]) {
   return JP_OK;
 }
 
+long jp_field_edge(jp_ctx* c) {
+  if (!c || c->kind == 2) return 0;
+  return c->kind == 1 ? c->amr.N2 : c->d.n;
+}
+
+int jp_field(jp_ctx* c, float* Q, float* speed, float* vel, long n3) {
+  if (!c || !Q || !speed) return JP_ERR_ARG;
+  if (c->kind == 2) return fail(c, JP_ERR_STATE, "output fields are per member (jp_ensemble_member)");
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_field before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  const int nf = int(jp_field_edge(c));
+  const size_t cells = size_t(nf) * nf * nf;
+  if (n3 < long(cells)) return fail(c, JP_ERR_ARG, "output arrays hold fewer than edge^3 entries");
+  if (int r = ensure_diag_buffers(c, nf)) return r;
+  const size_t need = cells * 5 * sizeof(float);
+  if (c->out_bytes < need) {
+    ctx_free(c, c->out);
+    c->out = nullptr;
+    c->out_bytes = 0;
+    void* p = nullptr;
+    if (int r = ctx_alloc(c, &p, need, "output field")) return r;
+    c->out = static_cast<float*>(p);
+    c->out_bytes = need;
+  }
+  if (c->kind == 1) {
+    if (int r = amr_composite(c, c->field)) return r;
+  } else {
+    HCK(c, dense_macro(c->prec, c->c0a, c->g0, float(1.0 / c->d.scale[0]), c->field, c->st));
+  }
+  float *dQ = c->out, *dS = c->out + cells, *dV = c->out + 2 * cells;
+  HCK(c, field_qspeed(c->prec == JP_PREC_FP64, c->kind == 0, c->field, nf, dQ, dS, vel ? dV : nullptr, c->st));
+  HCK(c, hipMemcpyAsync(Q, dQ, cells * 4, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipMemcpyAsync(speed, dS, cells * 4, hipMemcpyDeviceToHost, c->st));
+  if (vel) HCK(c, hipMemcpyAsync(vel, dV, cells * 12, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+
 int jp_get_derived(jp_ctx* c, jp_derived* d) {
   if (!c || !d) return JP_ERR_ARG;
   *d = c->d;
@@ -465,6 +503,7 @@ void jp_destroy(jp_ctx* c) {
   ens_destroy(c);
   for (auto& a : c->allocs) (void)hipFree(a.first);
   if (c->amr.hcnt) (void)hipHostFree(c->amr.hcnt);
+  if (c->host_out) (void)hipHostFree(c->host_out);
   if (c->amr.st2) {
     (void)hipStreamSynchronize(c->amr.st2);
     (void)hipStreamDestroy(c->amr.st2);

@@ -55,6 +55,10 @@ struct jp_ctx {
   void *c0a = nullptr, *c0b = nullptr;  // dense level (uniform lattice or AMR level 0)
   void* field = nullptr;
   size_t field_bytes = 0;
+  float* out = nullptr;  // output-field arrays (Q, speed, velocity) of the last jp_field
+  size_t out_bytes = 0;
+  float* host_out = nullptr;  // pinned host copy of them for the frame writer
+  size_t host_out_bytes = 0;
   double* partial = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<std::pair<void*, size_t>> allocs;

@@ -36,6 +36,9 @@ hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double
                       int variant, hipStream_t st);
 // macro field (rho, u, v, w): float4 for FP32/FP16/BF16 storage, double4 for FP64 storage
 hipError_t dense_macro(int prec, const void* f, const Dense& g, float dec, void* field, hipStream_t st);
+// Q-criterion, speed and (when vel != nullptr) velocity of an N^3 field of (rho, u, v, w) (field.hip)
+hipError_t field_qspeed(bool dbl_field, bool diff_in_field, const void* field, int N, float* Q, float* spd, float* vel,
+                        hipStream_t st);
 // E, enstrophy, mass sums of a macro field; diff_in_field_type selects the uniform driver's central
 // differences (taken in the field's type) over the AMR reduction's (taken in double)
 hipError_t field_diag(bool dbl_field, bool diff_in_field_type, const void* field, int N, double* partial,

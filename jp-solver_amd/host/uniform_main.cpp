@@ -36,15 +36,14 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "--prec must be fp32, fp16, bf16 or fp64\n");
     return 2;
   }
-  if (vtk) std::fprintf(stderr, "note: --vtk field output is not part of this build; running without it\n");
   jp_ctx* c = nullptr;
   if (int rc = jp_uniform_create(&p, &c)) return die(c, rc);
   jp_derived d;
   jp_get_derived(c, &d);
   const long steps = long(tstar / (p.u0 * d.kf) + 0.5);
   const int nb = (p.n / 8) * (p.n / 8) * (p.n / 8);
-  std::printf("[jp-solver uniform] TGV3D N=%d MB=8 blocks=%d Re=%g nu=%g tau=%.4f steps=%ld t*max=%.1f\n", p.n, nb, p.re,
-              d.nu, d.tau0, steps, tstar);
+  std::printf("[jp-solver uniform] TGV3D N=%d MB=8 blocks=%d Re=%g nu=%g tau=%.4f steps=%ld t*max=%.1f%s\n", p.n, nb, p.re,
+              d.nu, d.tau0, steps, tstar, vtk ? "  (writing vti)" : "");
   if (int rc = jp_init(c)) return die(c, rc);
   if (bench) {
     jp_bench_result r;
@@ -56,6 +55,9 @@ int main(int argc, char** argv) {
   }
   if (graph) jp_use_graph(c, 1);
   const long diagK = argi(argc, argv, "--diag-every", int(std::max(1L, steps / 240)));
+  const int frames = argi(argc, argv, "--frames", 160);  // src/tgv3d_uniform.cu:146, 165
+  const long frameK = vtk ? std::max(1L, steps / std::max(1, frames)) : 0;
+  int fi = 0;
   double E0 = -1, peak_ens = -1, peak_ts = -1, prevE = 1e30, mass0 = -1, mass_drift = 0;
   bool mono = true;
   long s = 0;
@@ -76,8 +78,14 @@ int main(int argc, char** argv) {
       std::printf("  t*=%.3f  E/E0=%.5f  enstrophy=%.6e  mass_drift=%.2e\n", ts, E / E0, ens, mass_drift);
       std::fflush(stdout);
     }
+    if (frameK && s % frameK == 0 && fi < frames) {
+      char nm[512];
+      std::snprintf(nm, sizeof nm, "%s_%04d.vti", vtk, fi++);
+      if (int rc = jp_write_vti(c, nm)) return die(c, rc);
+    }
     if (s >= steps) break;
-    const long next = std::min(steps, (s / diagK + 1) * diagK);
+    long next = std::min(steps, (s / diagK + 1) * diagK);
+    if (frameK) next = std::min(next, (s / frameK + 1) * frameK);
     if (int rc = jp_step(c, next - s)) return die(c, rc);
     s = next;
   }
@@ -86,6 +94,7 @@ int main(int argc, char** argv) {
   std::printf("  energy monotonic: %s   mass drift %.2e\n", mono ? "yes" : "NO", mass_drift);
   const bool ok = std::isfinite(peak_ens) && peak_ts > 4.0 && peak_ts < 10.0 && mono && mass_drift < 5e-4;
   std::printf("  %s\n", ok ? "VALID (enstrophy peaks then falls, energy monotonic, mass held in FP32)" : "CHECK (see values)");
+  if (vtk) std::printf("  wrote %d vti frames %s_0000.vti ..\n", fi, vtk);
   jp_destroy(c);
   return ok ? 0 : 1;
 }

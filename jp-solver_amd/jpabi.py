@@ -99,6 +99,8 @@ def lib():
             "jp_sync": (i, [vp]), "jp_last_error": (ctypes.c_char_p, [vp]), "jp_destroy": (None, [vp]),
             "jp_probe_copy": (i, [i, sz, i, ctypes.POINTER(d), ctypes.POINTER(d)]),
             "jp_probe_bandwidth": (i, [i, i, sz, i, ctypes.POINTER(d)]),
+            "jp_field_edge": (ctypes.c_long, [vp]), "jp_field": (i, [vp, vp, vp, vp, ctypes.c_long]),
+            "jp_write_vti": (i, [vp, ctypes.c_char_p]), "jp_write_boxes": (i, [vp, ctypes.c_char_p]),
             "jp_amr_create": (i, [ctypes.POINTER(AmrParams), ctypes.POINTER(vp)]),
             "jp_regrid": (i, [vp]), "jp_init_with_sensor": (i, [vp, vp, vp]),
             "jp_regrid_with_sensor": (i, [vp, vp, vp]),
@@ -178,6 +180,26 @@ class _Ctx:
         out = np.zeros(3)
         self._check(lib().jp_diag(self.h, out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def field(self, velocity=False):
+        """Q-criterion, speed (and velocity) of the output field, (edge, edge, edge) arrays [k, j, i]."""
+        n = lib().jp_field_edge(self.h)
+        q = np.empty(n ** 3, np.float32)
+        s = np.empty(n ** 3, np.float32)
+        v = np.empty(3 * n ** 3, np.float32) if velocity else None
+        vptr = v.ctypes.data_as(ctypes.c_void_p) if velocity else None
+        self._check(lib().jp_field(self.h, q.ctypes.data_as(ctypes.c_void_p), s.ctypes.data_as(ctypes.c_void_p), vptr,
+                                   n ** 3))
+        out = {"Q": q.reshape(n, n, n), "speed": s.reshape(n, n, n)}
+        if velocity:
+            out["velocity"] = v.reshape(n, n, n, 3)
+        return out
+
+    def write_vti(self, path):
+        self._check(lib().jp_write_vti(self.h, str(path).encode()))
+
+    def write_boxes(self, path):
+        self._check(lib().jp_write_boxes(self.h, str(path).encode()))
 
     def derived(self):
         d = Derived()

@@ -52,6 +52,8 @@ def lib():
             "orc_amr_level_elems": (l, [vp, i]), "orc_amr_elem": (l, [vp]),
             "orc_amr_get_raw": (None, [vp, i, vp]), "orc_amr_set_raw": (None, [vp, i, vp]),
             "orc_amr_get_f64": (None, [vp, i, vp]), "orc_amr_params": (None, [vp, vp]),
+            "orc_uni_qspeed": (None, [vp, vp, vp, vp]), "orc_amr_qspeed": (None, [vp, vp, vp]),
+            "orc_qspeed_field": (None, [vp, i, i, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -59,6 +61,20 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def qspeed_field(F, diff_in_field=True, velocity=True):
+    """Q, speed (and velocity) of an (N, N, N, 4) float32 field of (rho, u, v, w), index [k, j, i]."""
+    F = np.ascontiguousarray(F, dtype=np.float32)
+    n = F.shape[0]
+    q = np.empty(n ** 3, np.float32)
+    s = np.empty(n ** 3, np.float32)
+    v = np.empty(3 * n ** 3, np.float32) if velocity else None
+    lib().orc_qspeed_field(_ptr(F), n, 1 if diff_in_field else 0, _ptr(q), _ptr(s), _ptr(v) if velocity else None)
+    out = {"Q": q.reshape(n, n, n), "speed": s.reshape(n, n, n)}
+    if velocity:
+        out["velocity"] = v.reshape(n, n, n, 3)
+    return out
 
 
 def _ptr(a):
@@ -92,6 +108,13 @@ class Uniform:
 
     def step(self, n=1):
         lib().orc_uni_step(self.h, n)
+
+    def field(self):
+        """Q, speed, velocity of the output field (the uniform writer's arrays), index [k, j, i]."""
+        n = self.n
+        q, s, v = np.empty(n ** 3, np.float32), np.empty(n ** 3, np.float32), np.empty(3 * n ** 3, np.float32)
+        lib().orc_uni_qspeed(self.h, _ptr(q), _ptr(s), _ptr(v))
+        return {"Q": q.reshape(n, n, n), "speed": s.reshape(n, n, n), "velocity": v.reshape(n, n, n, 3)}
 
     def diag(self):
         out = np.zeros(3)
@@ -153,6 +176,13 @@ class Amr:
 
     def step(self, n=1):
         lib().orc_amr_step(self.h, n)
+
+    def field(self):
+        """Q and speed of the composite at 4 N0 (the AMR writer's arrays), index [k, j, i]."""
+        n = 4 * self.n
+        q, s = np.empty(n ** 3, np.float32), np.empty(n ** 3, np.float32)
+        lib().orc_amr_qspeed(self.h, _ptr(q), _ptr(s))
+        return {"Q": q.reshape(n, n, n), "speed": s.reshape(n, n, n)}
 
     def diag(self):
         out = np.zeros(3)

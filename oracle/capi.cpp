@@ -18,6 +18,7 @@ struct UniBase {
   virtual void init(const double* phase) = 0;
   virtual void step(long n) = 0;
   virtual void diag(double* out) = 0;
+  virtual void qspeed_out(float* Q, float* spd, float* vel) = 0;
   virtual size_t elem() const = 0;
   virtual void get_raw(void* dst) const = 0;
   virtual void set_raw(const void* src) = 0;
@@ -58,6 +59,11 @@ template <class S> struct Uni : UniBase {
     macro_uniform<S>(fa.data(), N, F.data(), sc);
     diag_uniform(F.data(), N, out);
   }
+  void qspeed_out(float* Q, float* spd, float* vel) override {  // src/tgv3d_uniform.cu:170, 110-119
+    std::vector<R> F(size_t(N) * N * N * 4);
+    macro_uniform<S>(fa.data(), N, F.data(), sc);
+    qspeed(F.data(), N, true, Q, spd, vel);
+  }
   void set_omega(double w) override { omega = R(w); }
   size_t elem() const override { return sizeof(raw); }
   void get_raw(void* dst) const override { std::memcpy(dst, fa.data(), fa.size() * sizeof(raw)); }
@@ -79,6 +85,7 @@ struct AmrBase {
   virtual int regrid(const float* sen0, const float* sen1pos) = 0;
   virtual void step(long n) = 0;
   virtual void diag(double* out) = 0;
+  virtual void qspeed_out(float* Q, float* spd) = 0;
   virtual void counts(long* out) const = 0;
   virtual int table(int which, void* dst, long cap) const = 0;
   virtual long level_elems(int level) const = 0;
@@ -105,6 +112,11 @@ template <class S> struct AmrO : AmrBase {
     for (long s = 0; s < n; ++s) a.base_step();
   }
   void diag(double* out) override { a.diag(out); }
+  void qspeed_out(float* Q, float* spd) override {  // src/tgv3d_amr.cu:871, 890
+    std::vector<typename S::real> F;
+    a.composite(F);
+    qspeed(F.data(), a.N2, false, Q, spd, nullptr);
+  }
   void counts(long* o) const override {
     long v[] = {a.cnb0, a.nref0, a.nf1, a.nref1, a.nf2, a.n_skin0, a.n_skin1, a.cap_nref0, a.cap1,
                 a.cap_nref1 == INT_MAX ? -1 : a.cap_nref1, a.cap2, a.hw_nf1, a.hw_nf2, a.overflow ? 1 : 0};
@@ -199,6 +211,9 @@ void orc_f32_to_bf16(const float* x, uint16_t* y, long n) {
 // ---- lattice core probes (pinned by identities in the tests) ----
 void orc_feq(int q, float rho, float ux, float uy, float uz, float* out) { *out = feq_q<float>(q, rho, ux, uy, uz); }
 void orc_regularize(float* n) { regularize<float>(n); }
+void orc_qspeed_field(const float* F, int N, int diff_in_field, float* Q, float* spd, float* vel) {
+  qspeed<float>(F, N, diff_in_field != 0, Q, spd, vel);
+}
 void orc_regularize64(double* n) { regularize<double>(n); }
 
 // ---- uniform context ----
@@ -216,6 +231,7 @@ void orc_uni_destroy(void* h) { delete static_cast<UniBase*>(h); }
 void orc_uni_init(void* h, const double* phase) { static_cast<UniBase*>(h)->init(phase); }
 void orc_uni_step(void* h, long n) { static_cast<UniBase*>(h)->step(n); }
 void orc_uni_diag(void* h, double* out) { static_cast<UniBase*>(h)->diag(out); }
+void orc_uni_qspeed(void* h, float* Q, float* spd, float* vel) { static_cast<UniBase*>(h)->qspeed_out(Q, spd, vel); }
 long orc_uni_elem(void* h) { return long(static_cast<UniBase*>(h)->elem()); }
 void orc_uni_get_raw(void* h, void* dst) { static_cast<UniBase*>(h)->get_raw(dst); }
 void orc_uni_set_raw(void* h, const void* src) { static_cast<UniBase*>(h)->set_raw(src); }
@@ -261,6 +277,7 @@ int orc_amr_regrid(void* h, const float* sen0, const float* sen1pos) {
 }
 void orc_amr_step(void* h, long n) { static_cast<AmrBase*>(h)->step(n); }
 void orc_amr_diag(void* h, double* out) { static_cast<AmrBase*>(h)->diag(out); }
+void orc_amr_qspeed(void* h, float* Q, float* spd) { static_cast<AmrBase*>(h)->qspeed_out(Q, spd); }
 void orc_amr_counts(void* h, long* out) { static_cast<AmrBase*>(h)->counts(out); }
 int orc_amr_table(void* h, int which, void* dst, long cap) { return static_cast<AmrBase*>(h)->table(which, dst, cap); }
 long orc_amr_level_elems(void* h, int level) { return static_cast<AmrBase*>(h)->level_elems(level); }

@@ -329,6 +329,40 @@ template <class R> void diag_uniform(const R* F, int N, double out[3]) {
   out[2] = mass * inv;
 }
 
+// Output field of a frame: Q-criterion, speed and velocity from an N^3 field (rho, u, v, w). The uniform
+// host writer (src/tgv3d_uniform.cu:96-119) differences two float field values in float before widening
+// (diff_in_field); the AMR device kernel k_qspeed (src/tgv3d_amr.cu:553-564) widens first.
+template <class R>
+void qspeed(const R* F, int N, bool diff_in_field, float* Qo, float* spd, float* vel) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        const int ip = (i + 1) % N, im = (i - 1 + N) % N, jp = (j + 1) % N, jm = (j - 1 + N) % N, kp = (k + 1) % N,
+                  km = (k - 1 + N) % N;
+        auto at = [&](int a, int b, int c, int comp) -> R { return F[((size_t(c) * N + b) * N + a) * 4 + 1 + comp]; };
+        auto d = [&](R a, R b) -> double { return diff_in_field ? double(R(a - b)) * 0.5 : (double(a) - double(b)) * 0.5; };
+        double g[3][3];
+        for (int c = 0; c < 3; ++c) {
+          g[c][0] = d(at(ip, j, k, c), at(im, j, k, c));
+          g[c][1] = d(at(i, jp, k, c), at(i, jm, k, c));
+          g[c][2] = d(at(i, j, kp, c), at(i, j, km, c));
+        }
+        double Q = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) Q -= 0.5 * g[a][b] * g[b][a];
+        const size_t p = (size_t(k) * N + j) * N + i;
+        const double u = F[p * 4 + 1], v = F[p * 4 + 2], w = F[p * 4 + 3];
+        Qo[p] = float(Q);
+        spd[p] = float(std::sqrt(u * u + v * v + w * w));
+        if (vel) {
+          vel[p * 3 + 0] = float(u);
+          vel[p * 3 + 1] = float(v);
+          vel[p * 3 + 2] = float(w);
+        }
+      }
+}
+
 // AMR diagnostics reduction, src/tgv3d_amr.cu:538-549: here the differences are taken in double.
 template <class R> void diag_composite(const R* F, int N, double out[3]) {
   std::vector<double> pe(N), pz(N), pm(N);
