@@ -151,6 +151,8 @@ def main():
         # a plain copy of known size in the same process calibrates the HBM byte counters
         jpabi.probe_copy(1 << 30, iters=2)
         fn = {"amr-dynamic": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps),
+              "amr-dynamic-graph": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps,
+                                                     graph=True),
               "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
               "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),

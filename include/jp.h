@@ -134,7 +134,10 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out);
 int jp_amr_create(const jp_amr_params* p, jp_ctx** out);
 
 /* AMR regrid: sensor, exact threshold, marks, scans, block tables, migration, skins, on the device;
- * only counts come back (src/tgv3d_amr.cu:722-764). JP_ERR_CAPACITY if a count exceeds its pool. */
+ * only counts come back (src/tgv3d_amr.cu:722-764). JP_ERR_CAPACITY if a count exceeds its pool.
+ * With graphs on (jp_use_graph) the regrid of a sensor-driven context runs on device-resident counts:
+ * nothing is read back and the host does not wait; a count beyond a pool is reported as JP_ERR_CAPACITY
+ * by the next call that reads the counts (jp_get_counts, jp_get_table, jp_diag, the population calls). */
 int jp_regrid(jp_ctx* c);
 
 /* Test hooks: the initial build / a regrid with the sensors replaced by the given values, so that two
@@ -183,9 +186,13 @@ int jp_init(jp_ctx* c);
  * 1:2:4 subcycle base_step (src/tgv3d_amr.cu:821-837). */
 int jp_step(jp_ctx* c, long n);
 
-/* When on, jp_step replays pairs of steps from a captured hipGraph (recaptured after any call that
- * changes the refinement or the buffers), as the reference's --graph production loop does
- * (src/tgv3d_amr.cu:872-881, 896). */
+/* When on, jp_step replays pairs of steps from captured hipGraphs, as the reference's --graph
+ * production loop does (src/tgv3d_amr.cu:872-881, 896). Pairs are captured once per role of the
+ * ping-pong buffers and kept. For a sensor-driven AMR context, turning graphs on also sizes the fine
+ * pools for the largest counts a regrid can produce (within the budget) and switches jp_regrid to
+ * device-resident counts: every level-1 and level-2 launch reads its tile count on the device, so the
+ * captured pairs stay valid across regrids and are not recaptured (the reference recaptures after each
+ * regrid). */
 int jp_use_graph(jp_ctx* c, int on);
 
 /* Launch variant of the dense-lattice step (not in the reference; a tuning knob): bits 0-1 select the
@@ -199,7 +206,10 @@ int jp_set_variant(jp_ctx* c, int variant);
 int jp_diag(jp_ctx* c, double out[3]);
 
 /* Pure step loop timed with HIP events on the context's stream (src/tgv3d_amr.cu:839-864; the uniform
- * reference has no bench path). use_graph replays the steps from a captured hipGraph. */
+ * reference has no bench path). A sensor-driven AMR context regrids every `adapt` steps in warm-up and
+ * timed loop. use_graph replays pairs of steps from captured hipGraphs; the AMR regrids then run on
+ * device-resident counts between replays (an ensemble replays without regrids, as the reference's graph
+ * bench does). */
 int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result* r);
 
 int jp_get_derived(jp_ctx* c, jp_derived* d);

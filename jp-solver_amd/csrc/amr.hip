@@ -10,52 +10,55 @@
 namespace jp {
 
 // ---- per-context kernels over the bodies above ----
-template <class S>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                    const typename S::real* __restrict__ gh, const int* __restrict__ nb,
-                                                   typename S::real om, float enc, float dec) {
+                                                   const int* __restrict__ dn, typename S::real om, float enc, float dec) {
   __shared__ int snb[kNbr];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
-  __syncthreads();
-  fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x);
+  for_tiles<kLoop>(dn, [=](int b, int lane) {
+    if (lane < kNbr) snb[lane] = nb[size_t(b) * kNbr + lane];
+    __syncthreads();
+    fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, lane);
+  });
 }
 // Last fine substep with the restriction into the parent level fused in (ptile/poct: the parent tile
 // and octant of each fine tile, i.e. l2pl1/l2oct or l1pcb/l1oct).
-template <class S, bool kToDense>
+template <class S, bool kToDense, bool kLoop>
 __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                      const typename S::real* __restrict__ gh, const int* __restrict__ nb,
-                                                     typename S::real om, float enc, float dec, typename S::raw* __restrict__ cf,
-                                                     const int* __restrict__ ptile, const int* __restrict__ poct, AmrGeo G,
-                                                     float nqi, float cenc) {
+                                                     const int* __restrict__ dn, typename S::real om, float enc, float dec,
+                                                     typename S::raw* __restrict__ cf, const int* __restrict__ ptile,
+                                                     const int* __restrict__ poct, AmrGeo G, float nqi, float cenc) {
   __shared__ int snb[kNbr];
   __shared__ RestrictLds<typename S::real> L;
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(b) * kNbr + threadIdx.x];
-  __syncthreads();
-  typename S::real kept[Q];
-  fine_step_cell<S, true>(fa, fb, gh, snb, om, enc, dec, b, threadIdx.x, kept);
-  restrict_tile<S, kToDense>(kept, threadIdx.x, L, cf, ptile[b], poct[b], G, nqi, cenc);
+  for_tiles<kLoop>(dn, [=](int b, int lane) {
+    if (lane < kNbr) snb[lane] = nb[size_t(b) * kNbr + lane];
+    __syncthreads();
+    typename S::real kept[Q];
+    fine_step_cell<S, true>(fa, fb, gh, snb, om, enc, dec, b, lane, kept);
+    restrict_tile<S, kToDense>(kept, lane, L, cf, ptile[b], poct[b], G, nqi, cenc);
+  });
 }
 
 // Ghost fills, one fine tile per workgroup (the parent box and prep values staged in LDS).
-template <class S>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(256) void k_fill_l1(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                                  const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
-                                                 const int* __restrict__ nb1, const uint32_t* __restrict__ slots, AmrGeo G,
-                                                 float nqs, float dec) {
+                                                 const int* __restrict__ nb1, const uint32_t* __restrict__ slots,
+                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec) {
   __shared__ FillLds<typename S::real> L;
-  fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, L);
+  for_tiles<kLoop, false>(dn, [=](int b, int lane) { fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, b, lane, L); });
 }
-template <class S>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                                  const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                                  const int* __restrict__ f1pos, const int* __restrict__ l1grid,
-                                                 const int* __restrict__ nb2, const uint32_t* __restrict__ slots, AmrGeo G,
-                                                 float nqs, float dec) {
+                                                 const int* __restrict__ nb2, const uint32_t* __restrict__ slots,
+                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec) {
   __shared__ FillLds<typename S::real> L;
-  fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, xcd_tile(blockIdx.x, gridDim.x),
-                  threadIdx.x, L);
+  for_tiles<kLoop, false>(dn, [=](int b, int lane) {
+    fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, b, lane, L);
+  });
 }
 // prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
 // non-equilibrium regularised and rescaled by nqs.
@@ -135,53 +138,56 @@ template <class S> __device__ inline void store_tile(typename S::raw* __restrict
 }
 
 // Seeding (src/tgv3d_amr.cu:299-308, 382-393) and migration (397-430).
-template <class S>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(512) void k_seed_l1(const typename S::raw* __restrict__ c0, typename S::raw* __restrict__ f1,
                                                  const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                                  const int* __restrict__ o_ref0, const int* __restrict__ o_fof0,
-                                                 const typename S::raw* __restrict__ oldf1, AmrGeo G, float nqs,
-                                                 float dec0, float enc1) {
-  const int fb = xcd_tile(blockIdx.x, gridDim.x), cell = threadIdx.x;
-  const int cb = l1pcb[fb], oct = l1oct[fb];
-  if (o_ref0 && o_ref0[cb]) {  // migration: the parent was refined before, copy its old tile
-    const int ofb = o_fof0[cb] + oct;
-    sfor<Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      f1[tidx(fb, q, cell)] = oldf1[tidx(ofb, q, cell)];
-    });
-    return;
-  }
-  const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
-  typename S::real out[Q];
-  prolong_l0<S>(c0, G, cbx * 2 * MB + (oct & 1) * MB + (cell & 7), cby * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7),
-                cbz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs, dec0, out);
-  store_tile<S>(f1, fb, cell, out, enc1);
+                                                 const typename S::raw* __restrict__ oldf1, const int* __restrict__ dn,
+                                                 AmrGeo G, float nqs, float dec0, float enc1) {
+  for_tiles<kLoop, false>(dn, [=](int fb, int cell) {
+    const int cb = l1pcb[fb], oct = l1oct[fb];
+    if (o_ref0 && o_ref0[cb]) {  // migration: the parent was refined before, copy its old tile
+      const int ofb = o_fof0[cb] + oct;
+      sfor<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f1[tidx(fb, q, cell)] = oldf1[tidx(ofb, q, cell)];
+      });
+      return;
+    }
+    const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
+    typename S::real out[Q];
+    prolong_l0<S>(c0, G, cbx * 2 * MB + (oct & 1) * MB + (cell & 7), cby * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7),
+                  cbz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs, dec0, out);
+    store_tile<S>(f1, fb, cell, out, enc1);
+  });
 }
-template <class S>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(512) void k_seed_l2(const typename S::raw* __restrict__ f1, typename S::raw* __restrict__ f2,
                                                  const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                                  const int* __restrict__ f1pos, const int* __restrict__ l1grid,
                                                  const int* __restrict__ o_l1grid, const int* __restrict__ o_ref1,
                                                  const int* __restrict__ o_fof1, const typename S::raw* __restrict__ oldf2,
-                                                 AmrGeo G, float nqs, float dec1, float enc2) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x), cell = threadIdx.x;
-  const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
-  if (o_l1grid) {  // migration keyed by the level-1 grid position
-    const int ol = o_l1grid[p];
-    if (ol >= 0 && o_ref1[ol]) {
-      const int of2 = o_fof1[ol] + oct;
-      sfor<Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f2[tidx(b, q, cell)] = oldf2[tidx(of2, q, cell)];
-      });
-      return;
+                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec1, float enc2) {
+  for_tiles<kLoop, false>(dn, [=](int b, int cell) {
+    const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
+    if (o_l1grid) {  // migration keyed by the level-1 grid position
+      const int ol = o_l1grid[p];
+      if (ol >= 0 && o_ref1[ol]) {
+        const int of2 = o_fof1[ol] + oct;
+        sfor<Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          f2[tidx(b, q, cell)] = oldf2[tidx(of2, q, cell)];
+        });
+        return;
+      }
     }
-  }
-  const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
-  typename S::real out[Q];
-  prolong_l1<S>(f1, l1grid, G, px * 2 * MB + (oct & 1) * MB + (cell & 7), py * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7),
-                pz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs, dec1, out);
-  store_tile<S>(f2, b, cell, out, enc2);
+    const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
+    typename S::real out[Q];
+    prolong_l1<S>(f1, l1grid, G, px * 2 * MB + (oct & 1) * MB + (cell & 7),
+                  py * 2 * MB + ((oct >> 1) & 1) * MB + ((cell >> 3) & 7), pz * 2 * MB + (oct >> 2) * MB + (cell >> 6), nqs,
+                  dec1, out);
+    store_tile<S>(f2, b, cell, out, enc2);
+  });
 }
 
 // Sensor (src/tgv3d_amr.cu:442-453): per tile, max over cells of sqrt(sum_q (f - feq)^2) / rho.
@@ -189,32 +195,33 @@ __device__ inline float wave_max(float v) {
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
   return v;
 }
-template <class S, bool kDense>
-__global__ __launch_bounds__(512) void k_sensor(const typename S::raw* __restrict__ f, AmrGeo G, float dec,
-                                                float* __restrict__ sen) {
+template <class S, bool kDense, bool kLoop>
+__global__ __launch_bounds__(512) void k_sensor(const typename S::raw* __restrict__ f, const int* __restrict__ dn, AmrGeo G,
+                                                float dec, float* __restrict__ sen) {
   using R = typename S::real;
   __shared__ float sm[8];
-  const int b = blockIdx.x, cell = threadIdx.x;
-  R v[Q];
-  if constexpr (kDense) load_dense<S>(f, G.g0.plane, dense_of_tile(G, b, cell), dec, v);
-  else load_tile<S>(f, b, cell, dec, v);
-  const Mom<R> m = moments(v);
-  const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
-  R s = R(0);
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    const R d2 = v[q] - feq<q>(e);
-    s += d2 * d2;
+  for_tiles<kLoop>(dn, [=](int b, int cell) {
+    R v[Q];
+    if constexpr (kDense) load_dense<S>(f, G.g0.plane, dense_of_tile(G, b, cell), dec, v);
+    else load_tile<S>(f, b, cell, dec, v);
+    const Mom<R> m = moments(v);
+    const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
+    R s = R(0);
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R d2 = v[q] - feq<q>(e);
+      s += d2 * d2;
+    });
+    float x = float(sqrt(s) / m.rho);
+    x = wave_max(x);
+    if ((cell & 63) == 0) sm[cell >> 6] = x;
+    __syncthreads();
+    if (cell == 0) {
+      float r = sm[0];
+      for (int w = 1; w < 8; ++w) r = fmaxf(r, sm[w]);
+      sen[b] = r;
+    }
   });
-  float x = float(sqrt(s) / m.rho);
-  x = wave_max(x);
-  if ((cell & 63) == 0) sm[cell >> 6] = x;
-  __syncthreads();
-  if (cell == 0) {
-    float r = sm[0];
-    for (int w = 1; w < 8; ++w) r = fmaxf(r, sm[w]);
-    sen[b] = r;
-  }
 }
 
 // Composite macro field at 4 N0 (src/tgv3d_amr.cu:515-536): level 2 where present, else level 1 at g/2,
@@ -261,78 +268,91 @@ __global__ __launch_bounds__(256) void k_composite(const typename S::raw* __rest
     default: return hipErrorInvalidValue;  \
   }
 
+// One-tile-per-workgroup instantiation for launches with a host count, looping one for device counts.
+#define JP_TILES(t, CALL)                            \
+  if ((t).count) {                                   \
+    constexpr bool kLoop = true;                     \
+    CALL;                                            \
+  } else {                                           \
+    constexpr bool kLoop = false;                    \
+    CALL;                                            \
+  }
+
 #define RAW(p) static_cast<const typename S::raw*>(p)
 #define WRAW(p) static_cast<typename S::raw*>(p)
 #define REAL(p) static_cast<const typename S::real*>(p)
 #define WREAL(p) static_cast<typename S::real*>(p)
 
-hipError_t amr_fine_step(int prec, int, const void* fa, void* fb, const void* gh, const int* nb, int ntiles, double omega,
+hipError_t amr_fine_step(int prec, int, const void* fa, void* fb, const void* gh, const int* nb, Tiles t, double omega,
                          float enc, float dec, hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step<S>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh), nb,
-                                       typename S::real(omega), enc, dec));
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh), nb,
+                                       t.count, typename S::real(omega), enc, dec)));
   return hipGetLastError();
 }
 
 hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
-                                  const int* nb, int ntiles, double omega, float enc, float dec, void* cf, const int* ptile,
+                                  const int* nb, Tiles t, double omega, float enc, float dec, void* cf, const int* ptile,
                                   const int* poct, float nqi, float cenc, hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
+  if (t.width <= 0) return hipSuccess;
   if (to_dense) {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step_r<S, true>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh),
-                                         nb, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G, nqi, cenc));
+    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step_r<S, true, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb),
+                                         REAL(gh), nb, t.count, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G,
+                                         nqi, cenc)));
   } else {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_fine_step_r<S, false>), dim3(ntiles), dim3(512), 0, st, RAW(fa), WRAW(fb), REAL(gh),
-                                         nb, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G, nqi, cenc));
+    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step_r<S, false, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb),
+                                         REAL(gh), nb, t.count, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G,
+                                         nqi, cenc)));
   }
   return hipGetLastError();
 }
 
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* nb1, const uint32_t* slots, int nf1, float nqs, float dec, hipStream_t st) {
-  if (nf1 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l1<S>), dim3(nf1), dim3(256), 0, st, RAW(c0), WREAL(gh1), l1pcb, l1oct, nb1,
-                                       slots, G, nqs, dec));
+                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, hipStream_t st) {
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_l1<S, kLoop>), dim3(t.width), dim3(256), 0, st, RAW(c0), WREAL(gh1), l1pcb, l1oct,
+                                       nb1, slots, t.count, G, nqs, dec)));
   return hipGetLastError();
 }
 
 hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, int nf2, float nqs,
+                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
                        float dec, hipStream_t st) {
-  if (nf2 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_fill_l2<S>), dim3(nf2), dim3(256), 0, st, RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos,
-                                       l1grid, nb2, slots, G, nqs, dec));
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_l2<S, kLoop>), dim3(t.width), dim3(256), 0, st, RAW(f1), WREAL(gh2), l2pl1, l2oct,
+                                       f1pos, l1grid, nb2, slots, t.count, G, nqs, dec)));
   return hipGetLastError();
 }
 
-hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct, int nf1,
+hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct, Tiles t,
                        float nqs, float dec0, float enc1, hipStream_t st) {
-  if (nf1 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l1<S>), dim3(nf1), dim3(512), 0, st, RAW(c0), WRAW(f1), l1pcb, l1oct,
-                                       nullptr, nullptr, nullptr, G, nqs, dec0, enc1));
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_seed_l1<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(c0), WRAW(f1), l1pcb, l1oct,
+                                       nullptr, nullptr, nullptr, t.count, G, nqs, dec0, enc1)));
   return hipGetLastError();
 }
 hipError_t amr_migrate_l1(int prec, const AmrGeo& G, const void* oldf1, void* newf1, const void* c0, const int* l1pcb,
-                          const int* l1oct, const int* o_ref0, const int* o_fof0, int nf1, float nqs, float dec0, float enc1,
+                          const int* l1oct, const int* o_ref0, const int* o_fof0, Tiles t, float nqs, float dec0, float enc1,
                           hipStream_t st) {
-  if (nf1 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l1<S>), dim3(nf1), dim3(512), 0, st, RAW(c0), WRAW(newf1), l1pcb, l1oct,
-                                       o_ref0, o_fof0, RAW(oldf1), G, nqs, dec0, enc1));
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_seed_l1<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(c0), WRAW(newf1), l1pcb, l1oct,
+                                       o_ref0, o_fof0, RAW(oldf1), t.count, G, nqs, dec0, enc1)));
   return hipGetLastError();
 }
 hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, int nf2, float nqs, float dec1, float enc2, hipStream_t st) {
-  if (nf2 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l2<S>), dim3(nf2), dim3(512), 0, st, RAW(f1), WRAW(f2), l2pl1, l2oct, f1pos,
-                                       l1grid, nullptr, nullptr, nullptr, nullptr, G, nqs, dec1, enc2));
+                       const int* f1pos, const int* l1grid, Tiles t, float nqs, float dec1, float enc2, hipStream_t st) {
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_seed_l2<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f1), WRAW(f2), l2pl1, l2oct,
+                                       f1pos, l1grid, nullptr, nullptr, nullptr, nullptr, t.count, G, nqs, dec1, enc2)));
   return hipGetLastError();
 }
 hipError_t amr_migrate_l2(int prec, const AmrGeo& G, const void* oldf2, void* newf2, const void* newf1, const int* l2pl1,
                           const int* l2oct, const int* f1pos, const int* o_l1grid, const int* o_ref1, const int* o_fof1,
-                          const int* l1grid, int nf2, float nqs, float dec1, float enc2, hipStream_t st) {
-  if (nf2 <= 0) return hipSuccess;
-  JP_DISPATCH(prec, hipLaunchKernelGGL((k_seed_l2<S>), dim3(nf2), dim3(512), 0, st, RAW(newf1), WRAW(newf2), l2pl1, l2oct,
-                                       f1pos, l1grid, o_l1grid, o_ref1, o_fof1, RAW(oldf2), G, nqs, dec1, enc2));
+                          const int* l1grid, Tiles t, float nqs, float dec1, float enc2, hipStream_t st) {
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_seed_l2<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(newf1), WRAW(newf2), l2pl1,
+                                       l2oct, f1pos, l1grid, o_l1grid, o_ref1, o_fof1, RAW(oldf2), t.count, G, nqs, dec1,
+                                       enc2)));
   return hipGetLastError();
 }
 
@@ -345,12 +365,12 @@ hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const 
   return hipGetLastError();
 }
 
-hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int ntiles, float dec, float* sen, hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
+hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen, hipStream_t st) {
+  if (t.width <= 0) return hipSuccess;
   if (level == 0) {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_sensor<S, true>), dim3(ntiles), dim3(512), 0, st, RAW(f), G, dec, sen));
+    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_sensor<S, true, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f), t.count, G, dec, sen)));
   } else {
-    JP_DISPATCH(prec, hipLaunchKernelGGL((k_sensor<S, false>), dim3(ntiles), dim3(512), 0, st, RAW(f), G, dec, sen));
+    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_sensor<S, false, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f), t.count, G, dec, sen)));
   }
   return hipGetLastError();
 }

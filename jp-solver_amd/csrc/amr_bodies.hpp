@@ -5,14 +5,14 @@
 
 namespace jp {
 
-__device__ inline int pmod(int a, int n) {
+__device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
 }
 // Trilinear base and weight of the fine coordinate g on the parent grid: x = (g + 1/2)/2 - 1/2,
 // base = floor(x), t = x - base (src/tgv3d_amr.cu:71-73). Exact in integers: g = 2m gives (m - 1, 3/4),
 // g = 2m + 1 gives (m, 1/4).
-__device__ inline void tri(int g, int& base, float& t) {
+__device__ __forceinline__ void tri(int g, int& base, float& t) {
   const int m = g >> 1;
   if (g & 1) {
     base = m;
@@ -22,33 +22,64 @@ __device__ inline void tri(int g, int& base, float& t) {
     t = 0.75f;
   }
 }
-__device__ inline uint32_t dense_cell(const AmrGeo& G, int ci, int cj, int ck) {
+__device__ __forceinline__ uint32_t dense_cell(const AmrGeo& G, int ci, int cj, int ck) {
   return (uint32_t(ck) * G.N0 + cj) * G.N0 + ci;
 }
-__device__ inline uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
+__device__ __forceinline__ uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
   const int bx = cb % G.cgx, by = (cb / G.cgx) % G.cgx, bz = cb / (G.cgx * G.cgx);
   return dense_cell(G, bx * MB + (cell & 7), by * MB + ((cell >> 3) & 7), bz * MB + (cell >> 6));
 }
-__device__ inline size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
+__device__ __forceinline__ size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
 
 // Tile of workgroup w in a launch of n one-tile workgroups. The dispatcher deals workgroups to the 8
 // XCDs round-robin (w -> XCD w % 8), and each XCD has its own L2: giving every XCD a contiguous run of
 // tiles keeps neighbouring tiles (the octants of one parent, whose faces and parent boxes overlap) in
 // one L2 instead of spreading them over all eight. A bijection of [0, n).
 constexpr int kXcds = 8;
-__device__ inline int xcd_tile(int w, int n) {
+__device__ __forceinline__ int xcd_tile(int w, int n) {
   const int x = w % kXcds, i = w / kXcds, q = n / kXcds, r = n % kXcds;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-template <class S> __device__ inline void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
+// The lane index as a value the compiler cannot see through, taken once per iteration of a loop over
+// tiles: address arithmetic derived from it is then computed inside the iteration instead of being
+// hoisted out of the loop and held in registers across it.
+__device__ __forceinline__ int opaque_lane() {
+  int lane = threadIdx.x;
+  asm volatile("" : "+v"(lane));
+  return lane;
+}
+
+// Tile loop of the per-tile kernels. Without kLoop the launch has one workgroup per tile and workgroup w
+// takes tile xcd_tile(w, width). With kLoop the tile count n is read on the device (*dn), so that a launch
+// recorded in a graph stays valid when a regrid changes it: workgroup w takes the indices v = w, w +
+// width, ... below n, each mapped to tile xcd_tile(v, n). The launch width is then a multiple of 8, so all
+// indices of a workgroup fall in its XCD's share, and the shares are balanced over the live count (a
+// mapping over the launch width would leave whole XCDs idle when the count is below it). With kSync, a
+// barrier precedes every index but the first, so that a body may rewrite the LDS the previous one used.
+// body(tile, lane) takes the lane index from the loop, in the looping form from opaque_lane(). The loop
+// still costs registers, which is why the one-tile form is a separate instantiation.
+template <bool kLoop, bool kSync = true, class F> __device__ __forceinline__ void for_tiles(const int* __restrict__ dn, F&& body) {
+  if constexpr (!kLoop) {
+    body(xcd_tile(blockIdx.x, gridDim.x), int(threadIdx.x));
+  } else {
+    const int n = *dn;
+    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+      if constexpr (kSync)
+        if (v != int(blockIdx.x)) __syncthreads();
+      body(xcd_tile(v, n), opaque_lane());
+    }
+  }
+}
+
+template <class S> __device__ __forceinline__ void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
                                                     typename S::real* out) {
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     out[q] = S::template load<q>(f[tidx(b, q, cell)], dec);
   });
 }
-template <class S> __device__ inline void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
+template <class S> __device__ __forceinline__ void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
                                                      float dec, typename S::real* out) {
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -60,7 +91,7 @@ template <class S> __device__ inline void load_dense(const typename S::raw* __re
 // (src/tgv3d_amr.cu:268-273). kept (optional, 19 registers): receives each stored population as it reads
 // back, for the fused restriction.
 template <class S, bool kKeep = false>
-__device__ inline void collide_store(typename S::real (&fin)[Q], typename S::real om, typename S::raw* __restrict__ fb,
+__device__ __forceinline__ void collide_store(typename S::real (&fin)[Q], typename S::real om, typename S::raw* __restrict__ fb,
                                      int b, int cell, float enc, typename S::real (&kept)[Q], float dec) {
   const Collider<S> col(fin);
   sfor<Q>([&](auto qc) {
@@ -86,7 +117,7 @@ template <class R> struct RestrictLds {
 // Parent cell (pi, pj, pk) of a fine tile's octant from its children's mean: feq + nqi * regularize(avg -
 // feq), stored to the parent tile pcb (or the dense level 0).
 template <class S, bool kToDense>
-__device__ inline void restrict_finish(typename S::real (&avg)[Q], typename S::raw* __restrict__ cf, int pcb, int oct, int pi,
+__device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], typename S::raw* __restrict__ cf, int pcb, int oct, int pi,
                                        int pj, int pk, const AmrGeo& G, float nqi, float cenc) {
   using R = typename S::real;
   R nq[Q];
@@ -109,7 +140,7 @@ __device__ inline void restrict_finish(typename S::real (&avg)[Q], typename S::r
   });
 }
 template <class S, bool kToDense>
-__device__ inline void restrict_tile(const typename S::real (&kept)[Q], int cell, RestrictLds<typename S::real>& L,
+__device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q], int cell, RestrictLds<typename S::real>& L,
                                      typename S::raw* __restrict__ cf, int pcb, int oct, const AmrGeo& G, float nqi,
                                      float cenc) {
   using R = typename S::real;
@@ -146,7 +177,7 @@ __device__ inline void restrict_tile(const typename S::real (&kept)[Q], int cell
 // to this level); missing sources come from the ghost record.
 // snb: the tile's neighbour entries, staged in LDS by the caller.
 template <class S, bool kKeep = false>
-__device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
   using R = typename S::real;
@@ -173,7 +204,7 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
 
 // The same without keeping the stored values.
 template <class S>
-__device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell) {
   typename S::real none[Q];
@@ -181,7 +212,7 @@ __device__ inline void fine_step_cell(const typename S::raw* __restrict__ fa, ty
 }
 
 // prep = feq + nqs * regularize(f - feq) of one cell, in place (src/tgv3d_amr.cu:160-171).
-template <class R> __device__ inline void prep_values(R* v, float nqs) {
+template <class R> __device__ __forceinline__ void prep_values(R* v, float nqs) {
   R nq[Q];
   const Mom<R> m = moments(v);
   const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
@@ -214,7 +245,7 @@ template <class R> struct FillLds {
 // Whether box cell (li, lj, lk) is a trilinear corner of some ghost slot. Slots toward a neighbour
 // offset d reach box coordinates {0, 1} for d = -1, {4, 5} for d = +1 and 0..5 for d = 0 (per axis); a
 // slot is a ghost only when that neighbour tile is absent (nb < 0).
-__device__ inline bool box_cell_needed(const int* nb, int li, int lj, int lk) {
+__device__ __forceinline__ bool box_cell_needed(const int* nb, int li, int lj, int lk) {
   const int xl = li < 2 ? -1 : 0, xh = li > 3 ? 1 : 0, yl = lj < 2 ? -1 : 0, yh = lj > 3 ? 1 : 0;
   const int zl = lk < 2 ? -1 : 0, zh = lk > 3 ? 1 : 0;
   for (int dz = zl; dz <= zh; ++dz)
@@ -224,14 +255,14 @@ __device__ inline bool box_cell_needed(const int* nb, int li, int lj, int lk) {
   return false;
 }
 // trilinear weight of corner c (bits di, dj, dk) for a fill-table entry's source parities
-__device__ inline float corner_weight(uint32_t e, int c) {
+__device__ __forceinline__ float corner_weight(uint32_t e, int c) {
   const float tx = (e >> 18) & 1 ? 0.25f : 0.75f, ty = (e >> 19) & 1 ? 0.25f : 0.75f, tz = (e >> 20) & 1 ? 0.25f : 0.75f;
   return ((c & 1) ? tx : 1 - tx) * ((c & 2) ? ty : 1 - ty) * ((c & 4) ? tz : 1 - tz);
 }
 
 // Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists).
 template <class S>
-__device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
+__device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                     const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                     const int* __restrict__ nb1, const uint32_t* __restrict__ slots, const AmrGeo& G,
                                     float nqs, float dec, int fb, int tid, FillLds<typename S::real>& L) {
@@ -274,7 +305,7 @@ __device__ inline void fill_l1_tile(const typename S::raw* __restrict__ c0, type
 // Level 2 from the scattered level 1: box cells without a level-1 tile are dropped and the remaining
 // weights renormalised (src/tgv3d_amr.cu:133-144, 187-202).
 template <class S>
-__device__ inline void fill_l2_tile(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
+__device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                     const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                     const int* __restrict__ f1pos, const int* __restrict__ l1grid,
                                     const int* __restrict__ nb2, const uint32_t* __restrict__ slots, const AmrGeo& G,

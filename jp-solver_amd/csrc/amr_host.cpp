@@ -28,6 +28,32 @@ static AmrGeo geo(const jp_ctx* c) {
 static size_t real_bytes(int prec) { return prec == JP_PREC_FP64 ? 8 : 4; }
 static float enc(const jp_ctx* c, int l) { return float(c->d.scale[l]); }
 static float dec(const jp_ctx* c, int l) { return float(1.0 / c->d.scale[l]); }
+// Launch width of the device-counted launches: the last count the host knew with 1/8 headroom for the
+// counts of later regrids, in multiples of 8 (workgroups beyond the live count return at once; a count
+// beyond the width gives some workgroups a second tile).
+static int width8(long n) { return int(std::max(8L, (n + n / 8 + 7) / 8 * 8)); }
+// A count beyond a pool leaves the grid incomplete: the context needs a new initial build before it
+// steps again.
+static int capacity_fail(jp_ctx* c, const std::string& msg) {
+  c->initialised = false;
+  return fail(c, JP_ERR_CAPACITY, msg);
+}
+
+// Tiles of the level-1 / level-2 launches: one workgroup per tile of the host's count, or, when regrids
+// run on device-resident counts, the count read on the device and a launch width from the count the host
+// last knew (width8; a launch recorded in a graph keeps its width, and the kernels loop over any tiles a
+// later regrid adds beyond it).
+static Tiles tiles(const jp_ctx* c, int level) {
+  const AmrState& a = c->amr;
+  if (!a.resident) return {nullptr, level == 1 ? a.nf1 : a.nf2};
+  return {a.dcnt + (level == 1 ? CNT_NF1 : CNT_NF2), level == 1 ? a.w1 : a.w2};
+}
+static Items items(const jp_ctx* c, int level) {
+  const AmrState& a = c->amr;
+  const int* dn = a.dcnt + (level == 1 ? CNT_NF1 : CNT_NF2);
+  return {a.resident ? (level == 1 ? a.w1 : a.w2) : (level == 1 ? a.nf1 : a.nf2), dn};
+}
+static Tiles tiles0(const jp_ctx* c) { return {nullptr, c->amr.cnb0}; }
 
 // Fill table of the compact ghost record, one entry per slot (encoding in amr_internal.hpp). Every fine
 // tile's origin is even on its own grid, so the box offset and the weights depend on the slot only.
@@ -56,6 +82,27 @@ static int read_counts(jp_ctx* c) {
 }
 static int fill(jp_ctx* c, int* p, int v, size_t n) {
   HCK(c, fill_i32(p, v, n, c->st));
+  return JP_OK;
+}
+// Host copies of the counts after device-resident regrids, read when a caller needs them. The overflow
+// flag is sticky: a resident regrid whose counts exceeded a pool left the grid incomplete.
+int amr_sync_counts(jp_ctx* c) {
+  AmrState& a = c->amr;
+  if (!a.counts_stale) return JP_OK;
+  if (int r = read_counts(c)) return r;
+  a.counts_stale = false;
+  a.nref0 = a.hcnt[CNT_NREF0];
+  a.nf1 = a.hcnt[CNT_NF1];
+  a.nref1 = a.hcnt[CNT_NREF1];
+  a.nf2 = a.hcnt[CNT_NF2];
+  a.hw_nf1 = std::max(a.hw_nf1, long(a.hcnt[CNT_HW1]));
+  a.hw_nf2 = std::max(a.hw_nf2, long(a.hcnt[CNT_HW2]));
+  a.w1 = width8(a.nf1);
+  a.w2 = width8(a.nf2);
+  if (a.hcnt[CNT_OVERFLOW])
+    return capacity_fail(c, "a device-resident regrid exceeded the pool capacity (level-1 " +
+                                        std::to_string(8L * a.nref0) + "/" + std::to_string(a.cap1) + ", level-2 " +
+                                        std::to_string(8L * a.nref1) + "/" + std::to_string(a.cap2) + "); raise --maxgb");
   return JP_OK;
 }
 
@@ -93,19 +140,20 @@ static void free_l2_pool(jp_ctx* c) {
 // prep where they need it), so they are built on demand by ensure_skins.
 static int finish_tables(jp_ctx* c) {
   AmrState& a = c->amr;
-  if (a.nf1) HCK(c, rg_nbr_l1(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.nb1, c->st));
-  if (a.nf2) HCK(c, rg_nbr_l2(a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.fof1, a.l1gx, a.nf2, a.nb2, c->st));
+  HCK(c, rg_nbr_l1(a.f1pos, a.l1grid, a.l1gx, items(c, 1), a.nb1, c->st));
+  HCK(c, rg_nbr_l2(a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.ref1, a.fof1, a.l1gx, items(c, 2), a.nb2, c->st));
   a.skins_stale = true;
   return JP_OK;
 }
 static int ensure_skins(jp_ctx* c) {
   AmrState& a = c->amr;
+  if (int r = amr_sync_counts(c)) return r;
   if (!a.skins_stale) return JP_OK;
   HCK(c, rg_dilate_l0(a.ref0, a.cnb0, a.cgx, a.flags, c->st));
-  HCK(c, rg_compact(a.flags, a.cnb0, a.skin0, a.dcnt, CNT_SKIN0, c->st));
+  HCK(c, rg_compact(a.flags, {a.cnb0, nullptr}, a.skin0, a.dcnt, CNT_SKIN0, c->st));
   if (a.nf1) {
-    HCK(c, rg_dilate_l1(a.ref1, a.f1pos, a.l1grid, a.nf1, a.l1gx, a.flags, c->st));
-    HCK(c, rg_compact(a.flags, a.nf1, a.skin1, a.dcnt, CNT_SKIN1, c->st));
+    HCK(c, rg_dilate_l1(a.ref1, a.f1pos, a.l1grid, {a.nf1, nullptr}, a.l1gx, a.flags, c->st));
+    HCK(c, rg_compact(a.flags, {a.nf1, nullptr}, a.skin1, a.dcnt, CNT_SKIN1, c->st));
   }
   if (int r = read_counts(c)) return r;
   a.n_skin0 = a.hcnt[CNT_SKIN0];
@@ -135,7 +183,7 @@ static int grow_l1(jp_ctx* c, bool keep_f1a) {
   int cn = std::min(a.cnb0, std::max(a.nref0, (a.cap1 / 8) * 3 / 2));
   while (cn > a.nref0 && !pool_fits(a, 8L * cn, c2)) cn--;
   if (!pool_fits(a, 8L * cn, c2))
-    return fail(c, JP_ERR_CAPACITY, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
+    return capacity_fail(c, "level-1 count " + std::to_string(a.nf1) + " exceeds the pool capacity " +
                                         std::to_string(a.cap1) + " and the budget (threshold ties); raise --maxgb");
   HCK(c, hipStreamSynchronize(c->st));
   for (void* p : {a.f1b, a.gh1}) ctx_free(c, p);
@@ -146,10 +194,10 @@ static int grow_l1(jp_ctx* c, bool keep_f1a) {
 }
 
 // level-1 marks -> scan -> counts -> tables (common to init and regrid). ref0 is already marked.
-static int scatter_l1(jp_ctx* c) {
+static int scatter_l1(jp_ctx* c, bool clamp = false) {
   AmrState& a = c->amr;
   if (int r = fill(c, a.l1grid, -1, size_t(a.NG1))) return r;  // the scatter writes every fof0 entry
-  HCK(c, rg_scatter_l1(a.ref0, a.rank, a.cnb0, a.cgx, a.l1gx, a.cap1, a.fof0, a.rl0, a.l1pcb, a.l1oct, a.l1grid, a.f1pos,
+  HCK(c, rg_scatter_l1(a.ref0, a.rank, a.cnb0, a.cgx, a.l1gx, a.cap1, clamp, a.fof0, a.rl0, a.l1pcb, a.l1oct, a.l1grid, a.f1pos,
                        a.dcnt, c->st));
   return JP_OK;
 }
@@ -159,7 +207,7 @@ static int scatter_l1(jp_ctx* c) {
 // case that the count exceeds the pool, the pool grows and the scatter is repeated.
 static int build_l1_tables(jp_ctx* c, bool keep_f1a) {
   AmrState& a = c->amr;
-  HCK(c, rg_scan(a.ref0, a.cnb0, a.rank, a.dcnt, CNT_NREF0, c->st));
+  HCK(c, rg_scan(a.ref0, {a.cnb0, nullptr}, a.rank, a.dcnt, CNT_NREF0, CNT_NF1, a.cap1, c->st));
   if (int r = scatter_l1(c)) return r;
   if (int r = read_counts(c)) return r;
   a.nref0 = a.hcnt[CNT_NREF0];
@@ -168,16 +216,17 @@ static int build_l1_tables(jp_ctx* c, bool keep_f1a) {
   if (a.nf1 > a.cap1) {
     if (int r = grow_l1(c, keep_f1a)) return r;
     if (int r = fill(c, a.dcnt + CNT_OVERFLOW, 0, 1)) return r;
+    if (int r = fill(c, a.dcnt + CNT_NF1, a.nf1, 1)) return r;  // the scan stopped the count at the old pool
     a.hcnt[CNT_OVERFLOW] = 0;
     if (int r = scatter_l1(c)) return r;
   }
   return JP_OK;
 }
 
-static int scatter_l2(jp_ctx* c) {
+static int scatter_l2(jp_ctx* c, bool clamp = false) {
   AmrState& a = c->amr;
   if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
-  HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
+  HCK(c, rg_scatter_l2(a.ref1, a.rank, items(c, 1), a.cap2, clamp, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
   return JP_OK;
 }
 
@@ -185,7 +234,8 @@ static int scatter_l2(jp_ctx* c) {
 // when the pool exists, and read the level-2 count.
 static int scan_l2(jp_ctx* c, bool scatter) {
   AmrState& a = c->amr;
-  HCK(c, rg_scan(a.ref1, a.nf1, a.rank, a.dcnt, CNT_NREF1, c->st));
+  // before the pool exists (init) the count is not limited here; init checks it against the pool
+  HCK(c, rg_scan(a.ref1, items(c, 1), a.rank, a.dcnt, CNT_NREF1, CNT_NF2, scatter ? a.cap2 : INT_MAX, c->st));
   if (scatter)
     if (int r = scatter_l2(c)) return r;
   if (int r = read_counts(c)) return r;
@@ -201,14 +251,16 @@ static int upload_sensor(jp_ctx* c, const float* sen0, int n) {
 }
 static int sensor_l1(jp_ctx* c, const float* sen1_by_pos) {
   AmrState& a = c->amr;
-  HCK(c, amr_sensor(c->prec, 1, geo(c), a.f1a, a.nf1, dec(c, 1), a.sen1, c->st));
+  HCK(c, amr_sensor(c->prec, 1, geo(c), a.f1a, tiles(c, 1), dec(c, 1), a.sen1, c->st));
   if (sen1_by_pos) {
     HCK(c, hipMemcpyAsync(a.sen1pos, sen1_by_pos, size_t(a.NG1) * 4, hipMemcpyHostToDevice, c->st));
-    HCK(c, rg_gather_by_pos(a.sen1pos, a.f1pos, a.nf1, a.sen1, c->st));
+    HCK(c, rg_gather_by_pos(a.sen1pos, a.f1pos, items(c, 1), a.sen1, c->st));
   }
-  HCK(c, rg_deepmask(a.f1pos, a.l1grid, a.l1gx, a.nf1, a.deep, c->st));
+  HCK(c, rg_deepmask(a.f1pos, a.l1grid, a.l1gx, items(c, 1), a.deep, c->st));
   return JP_OK;
 }
+
+static int apply_resident(jp_ctx* c);
 
 // ---- initial build, src/tgv3d_amr.cu:776-815 ----
 int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
@@ -218,6 +270,7 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   HCK(c, dense_init(c->prec, c->c0a, c->g0, c->u0, c->d.kf, c->phase, enc(c, 0), c->st));
   if (int r = fill(c, a.dcnt, 0, CNT_N)) return r;
   a.hw_nf1 = a.hw_nf2 = 0;
+  a.counts_stale = false;
   if (a.l1_mode == JP_MODE_ALL) HCK(c, rg_set(a.ref0, a.cnb0, 0, a.cgx, nullptr, c->st));
   else if (a.l1_mode == JP_MODE_SLAB) HCK(c, rg_set(a.ref0, a.cnb0, 1, a.cgx, nullptr, c->st));
   else {
@@ -225,14 +278,14 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       HCK(c, dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, c->st));
       std::swap(c->c0a, c->c0b);
     }
-    HCK(c, amr_sensor(c->prec, 0, G, c->c0a, a.cnb0, dec(c, 0), a.sen0, c->st));
+    HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
     if (sen0)
       if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
-    HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
+    HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
     HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
   }
   if (int r = build_l1_tables(c, false)) return r;
-  if (a.nf1) HCK(c, amr_seed_l1(c->prec, G, c->c0a, a.f1a, a.l1pcb, a.l1oct, a.nf1, float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
+  HCK(c, amr_seed_l1(c->prec, G, c->c0a, a.f1a, a.l1pcb, a.l1oct, tiles(c, 1), float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
   a.nref1 = a.nf2 = 0;
   if (a.nf1) {
     if (int r = sensor_l1(c, sen1_by_pos)) return r;
@@ -240,8 +293,8 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     if (a.l2_mode == JP_MODE_ALL) HCK(c, rg_set(a.ref1, a.nf1, 2, a.cgx, a.deep, c->st));
     else {
       // init does not cap the level-2 target (src/tgv3d_amr.cu:795)
-      HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, INT_MAX, a.thr, a.dcnt, CNT_NDEEP, c->st));
-      HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
+      HCK(c, rg_select(a.sen1, a.deep, items(c, 1), a.frac2, INT_MAX, a.thr, a.dcnt, CNT_NDEEP, c->st));
+      HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, items(c, 1), c->st));
     }
     if (int r = scan_l2(c, false)) return r;
     a.cap_nref1 = std::max(1, int(std::ceil(a.nref1 * 2.5)));
@@ -259,46 +312,85 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   }
   if (!a.f2a)
     if (int r = alloc_l2_pool(c)) return r;
+  if (a.resident) a.cap2 = a.cap2_alloc;  // a re-initialisation keeps the pools sized for resident regrids
   if (a.nf2 > a.cap2)
-    return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
+    return capacity_fail(c, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
                                         std::to_string(a.cap2) + "; raise --maxgb");
   if (int r = fill(c, a.fof1, -1, size_t(a.cap1))) return r;
   if (a.nf2) {
-    HCK(c, rg_scatter_l2(a.ref1, a.rank, a.nf1, a.cap2, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
-    HCK(c, amr_seed_l2(c->prec, G, a.f1a, a.f2a, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1),
-                       enc(c, 2), c->st));
+    HCK(c, rg_scatter_l2(a.ref1, a.rank, items(c, 1), a.cap2, false, a.fof1, a.rl1, a.l2pl1, a.l2oct, a.dcnt, c->st));
+    HCK(c, amr_seed_l2(c->prec, G, a.f1a, a.f2a, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, tiles(c, 2), float(c->d.nqs12),
+                       dec(c, 1), enc(c, 2), c->st));
   }
   if (int r = finish_tables(c)) return r;
-  if (a.hcnt[CNT_OVERFLOW]) return fail(c, JP_ERR_CAPACITY, "pool capacity exceeded while scattering tables");
+  if (a.hcnt[CNT_OVERFLOW]) return capacity_fail(c, "pool capacity exceeded while scattering tables");
   c->initialised = true;
-  return JP_OK;
+  a.w1 = width8(a.nf1);
+  a.w2 = width8(a.nf2);
+  return a.resident_req && !a.resident ? apply_resident(c) : JP_OK;
 }
 int amr_init(jp_ctx* c) { return amr_init_with(c, nullptr, nullptr); }
 
 // ---- regrid, src/tgv3d_amr.cu:722-764 ----
+// snapshots of the tables the migration keys on, one launch
+static int snapshot_tables(jp_ctx* c) {
+  AmrState& a = c->amr;
+  const int* src[5] = {a.ref0, a.fof0, a.l1grid, a.ref1, a.fof1};
+  int* dst[5] = {a.o_ref0, a.o_fof0, a.o_l1grid, a.o_ref1, a.o_fof1};
+  const long n[5] = {a.cnb0, a.cnb0, a.NG1, a.cap1, a.cap1};
+  HCK(c, rg_copy_i32_multi(src, dst, n, 5, c->st));
+  return JP_OK;
+}
+
+// The regrid on device-resident counts: the same pipeline with every level-1 and level-2 launch sized
+// on the device, no count read back and no pool growth (the pools hold the largest counts a regrid can
+// produce, see make_resident). Both fine levels always migrate, so each regrid swaps both buffer pairs
+// and the captured steps of either parity stay valid.
+static int regrid_resident(jp_ctx* c) {
+  AmrState& a = c->amr;
+  const AmrGeo G = geo(c);
+  if (int r = snapshot_tables(c)) return r;
+  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
+  HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
+  HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
+  HCK(c, rg_scan(a.ref0, {a.cnb0, nullptr}, a.rank, a.dcnt, CNT_NREF0, CNT_NF1, a.cap1, c->st));
+  if (int r = scatter_l1(c, true)) return r;
+  HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, tiles(c, 1),
+                        float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
+  std::swap(a.f1a, a.f1b);
+  if (int r = sensor_l1(c, nullptr)) return r;
+  HCK(c, rg_select(a.sen1, a.deep, items(c, 1), a.frac2, a.cap_nref1, a.thr, a.dcnt, CNT_NDEEP, c->st));
+  if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
+  HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, items(c, 1), c->st));
+  HCK(c, rg_scan(a.ref1, items(c, 1), a.rank, a.dcnt, CNT_NREF1, CNT_NF2, a.cap2, c->st));
+  if (int r = scatter_l2(c, true)) return r;
+  HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
+                        a.l1grid, tiles(c, 2), float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
+  std::swap(a.f2a, a.f2b);
+  a.counts_stale = true;
+  return finish_tables(c);
+}
+
 int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
+  if (a.resident && !sen0 && !sen1_by_pos) return regrid_resident(c);
+  if (int r = amr_sync_counts(c)) return r;
   drop_graph(c);
-  {  // snapshots of the tables the migration keys on, one launch
-    const int* src[5] = {a.ref0, a.fof0, a.l1grid, a.ref1, a.fof1};
-    int* dst[5] = {a.o_ref0, a.o_fof0, a.o_l1grid, a.o_ref1, a.o_fof1};
-    const long n[5] = {a.cnb0, a.cnb0, a.NG1, a.cap1, a.cap1};
-    HCK(c, rg_copy_i32_multi(src, dst, n, 5, c->st));
-  }
-  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, a.cnb0, dec(c, 0), a.sen0, c->st));
+  if (int r = snapshot_tables(c)) return r;
+  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
   if (sen0)
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
-  HCK(c, rg_select(a.sen0, nullptr, a.cnb0, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
+  HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
   HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
-  HCK(c, rg_differs(a.ref0, a.o_ref0, a.cnb0, a.dcnt, CNT_DIFF0, c->st));
+  HCK(c, rg_differs(a.ref0, a.o_ref0, {a.cnb0, nullptr}, a.dcnt, CNT_DIFF0, c->st));
   if (int r = build_l1_tables(c, true)) return r;
   // Unchanged level-0 marks give the same level-1 tiles in the same order: the migration would be an
   // identity copy and is skipped (likewise for level 2 below).
   const bool same1 = !a.hcnt[CNT_DIFF0] && !a.f1_stale;
   if (a.nf1 && !same1) {
-    HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, a.nf1, float(c->d.nqs01),
-                          dec(c, 0), enc(c, 1), c->st));
+    HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, tiles(c, 1),
+                          float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
     std::swap(a.f1a, a.f1b);
   }
   if (a.f1_stale) {  // the pool grew: the buffer that held the old level-1 data still has the old size
@@ -311,10 +403,10 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   a.nref1 = a.nf2 = 0;
   if (a.nf1) {
     if (int r = sensor_l1(c, sen1_by_pos)) return r;
-    HCK(c, rg_select(a.sen1, a.deep, a.nf1, a.frac2, a.cap_nref1, a.thr, a.dcnt, CNT_NDEEP, c->st));
+    HCK(c, rg_select(a.sen1, a.deep, items(c, 1), a.frac2, a.cap_nref1, a.thr, a.dcnt, CNT_NDEEP, c->st));
     if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
-    HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, a.nf1, c->st));
-    HCK(c, rg_differs(a.ref1, a.o_ref1, a.nf1, a.dcnt, CNT_DIFF1, c->st));
+    HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, items(c, 1), c->st));
+    HCK(c, rg_differs(a.ref1, a.o_ref1, items(c, 1), a.dcnt, CNT_DIFF1, c->st));
     if (int r = scan_l2(c, true)) return r;
     // Ties at the threshold can exceed the level-2 pool (the reference writes out of bounds there): the
     // pool grows when the budget allows; the refinement target cap_nref1 stays as measured at init.
@@ -323,7 +415,7 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       int cn = std::max(a.nref1, (a.cap2 / 8) * 3 / 2);
       while (cn > a.nref1 && !pool_fits(a, a.cap1, 8L * cn)) cn--;
       if (!pool_fits(a, a.cap1, 8L * cn))
-        return fail(c, JP_ERR_CAPACITY, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
+        return capacity_fail(c, "level-2 count " + std::to_string(a.nf2) + " exceeds the pool capacity " +
                                             std::to_string(a.cap2) + " and the budget (threshold ties); raise --maxgb");
       HCK(c, hipStreamSynchronize(c->st));
       void* keep = a.f2a;  // old level-2 data, read by the migration below
@@ -335,14 +427,15 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
       a.f2a = keep;
       grown = true;
     }
-    if (grown) {  // the scatter ahead of the count read stopped at the old pool
+    if (grown) {  // the scan and the scatter ahead of the count read stopped at the old pool
       if (int r = fill(c, a.dcnt + CNT_OVERFLOW, 0, 1)) return r;
+      if (int r = fill(c, a.dcnt + CNT_NF2, a.nf2, 1)) return r;
       a.hcnt[CNT_OVERFLOW] = 0;
       if (int r = scatter_l2(c)) return r;
     }
     if (a.nf2 && !(same1 && !a.hcnt[CNT_DIFF1] && !grown)) {
       HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
-                            a.l1grid, a.nf2, float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
+                            a.l1grid, tiles(c, 2), float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
       std::swap(a.f2a, a.f2b);
     }
     if (grown) {  // the other buffer still has the old size
@@ -353,8 +446,77 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
     }
   }
   if (int r = finish_tables(c)) return r;
-  if (a.hcnt[CNT_OVERFLOW]) return fail(c, JP_ERR_CAPACITY, "pool capacity exceeded while scattering tables");
+  if (a.hcnt[CNT_OVERFLOW]) return capacity_fail(c, "pool capacity exceeded while scattering tables");
+  a.w1 = width8(a.nf1);
+  a.w2 = width8(a.nf2);
   return JP_OK;
+}
+
+// ---- device-resident regrids ----
+// A buffer replaced by a larger one, its first `keep` bytes carried over.
+static int realloc_keep(jp_ctx* c, void** p, size_t bytes, size_t keep, const char* what) {
+  void* n = nullptr;
+  if (int r = ctx_alloc(c, &n, bytes, what)) return r;
+  if (keep) HCK(c, hipMemcpyAsync(n, *p, keep, hipMemcpyDeviceToDevice, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  ctx_free(c, *p);
+  *p = n;
+  return JP_OK;
+}
+template <class T> static int realloc_keep_t(jp_ctx* c, T** p, size_t count, size_t keep, const char* what) {
+  void* v = *p;
+  int r = realloc_keep(c, &v, count * sizeof(T), keep * sizeof(T), what);
+  *p = static_cast<T*>(v);
+  return r;
+}
+
+// A regrid on device-resident counts cannot grow a pool (the host learns the counts only afterwards),
+// so the fine pools are sized once for the largest counts a regrid can produce: every level-0 tile
+// refined (8 cnb0 level-1 tiles) and every level-1 tile refined (8 level-2 tiles each), as far as the
+// budget allows. The current populations and level-2 tables are carried over; ghost records and the
+// second buffers are rewritten before they are read. Counts beyond a budget-limited pool still raise the
+// sticky overflow flag, which the next count read reports.
+static int apply_resident(jp_ctx* c) {
+  AmrState& a = c->amr;
+  if (int r = amr_sync_counts(c)) return r;
+  a.resident = false;
+  if (!a.resident_req || !amr_dynamic(c)) return JP_OK;
+  const double per_tile = double(Q) * 4 * (2 * MB3 + 1000) * 6 / 5;  // pool_fits' bytes per fine tile
+  const long room = std::max(0L, long(a.budget / per_tile) - 1);     // fine tiles of both levels
+  const long c1 = std::max<long>(a.cap1, std::min<long>(8L * a.cnb0, (room - a.cap2_alloc) / 8 * 8));
+  const long c2 = std::max<long>(a.cap2_alloc, std::min<long>(8L * c1, (room - c1) / 8 * 8));
+  const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
+  if (c1 > a.cap1) {
+    if (int rc = realloc_keep(c, &a.f1a, size_t(c1) * tile * e, size_t(a.nf1) * tile * e, "f1a")) return rc;
+    if (int rc = realloc_keep(c, &a.f1b, size_t(c1) * tile * e, 0, "f1b")) return rc;
+    if (int rc = realloc_keep(c, &a.gh1, size_t(c1) * kGhostSlots * r, 0, "gh1")) return rc;
+    a.cap1 = int(c1);
+    ++a.pool_gen;
+  }
+  if (c2 > a.cap2_alloc) {
+    if (int rc = realloc_keep(c, &a.f2a, size_t(c2) * tile * e, size_t(a.nf2) * tile * e, "f2a")) return rc;
+    if (int rc = realloc_keep(c, &a.f2b, size_t(c2) * tile * e, 0, "f2b")) return rc;
+    if (int rc = realloc_keep(c, &a.gh2, size_t(c2) * kGhostSlots * r, 0, "gh2")) return rc;
+    if (int rc = realloc_keep_t(c, &a.l2pl1, size_t(c2), size_t(a.nf2), "l2pl1")) return rc;
+    if (int rc = realloc_keep_t(c, &a.l2oct, size_t(c2), size_t(a.nf2), "l2oct")) return rc;
+    if (int rc = realloc_keep_t(c, &a.nb2, size_t(c2) * kNbr, size_t(a.nf2) * kNbr, "nb2")) return rc;
+    a.cap2_alloc = int(c2);
+    ++a.pool_gen;
+  }
+  a.cap2 = a.cap2_alloc;
+  a.resident = true;
+  a.w1 = width8(a.nf1);
+  a.w2 = width8(a.nf2);
+  drop_graph(c);  // launches recorded before sized their level-2 work on the host's counts
+  return JP_OK;
+}
+hipError_t amr_log_counts(jp_ctx* c, int* dst) {
+  static_assert(CNT_NF2 == CNT_NF1 + 1, "the two tile counts are adjacent");
+  return hipMemcpyAsync(dst, c->amr.dcnt + CNT_NF1, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st);
+}
+int amr_set_resident(jp_ctx* c, bool on) {
+  c->amr.resident_req = on;
+  return c->initialised ? apply_resident(c) : JP_OK;  // else applied by the initial build
 }
 
 // ---- one coarse step of the nested 1:2:4 subcycle, src/tgv3d_amr.cu:821-837 ----
@@ -363,15 +525,16 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
 static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
   AmrState& a = c->amr;
   const int p = c->prec;
+  const Tiles t2 = tiles(c, 2);
   if (!a.skipprep)
-    HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, a.nf2, float(c->d.nqs12),
+    HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12),
                        dec(c, 1), st));
-  HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+  HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
   std::swap(a.f2a, a.f2b);
   if (a.skiprestr)
-    HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+    HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
   else
-    HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, a.nf2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a,
+    HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a,
                                   a.l2pl1, a.l2oct, float(c->d.nqi21), enc(c, 1), st));
   std::swap(a.f2a, a.f2b);
   return JP_OK;
@@ -382,12 +545,14 @@ static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
 // level-1 fill touch only level 0 and the level-1 ghost record: the pair runs on a second stream,
 // overlapping them, and the first level-1 substep waits for both. Every buffer is written by at most one
 // of the two branches and read by the other only after the join, so the results are those of the
-// sequential order.
+// sequential order. With device-resident regrids the host does not know whether a level is empty, so
+// every launch is made (an empty level's workgroups read a zero count and return).
 int amr_base_step(jp_ctx* c, hipStream_t st) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   const int p = c->prec;
-  const bool fork = a.nf2 > 0;
+  const bool fork = a.resident || a.nf2 > 0;
+  const Tiles t1 = tiles(c, 1);
   hipEvent_t ev_fork = a.ev_fork[a.evi], ev_join = a.ev_join[a.evi];
   a.evi ^= 1;  // two event pairs: consecutive steps in one captured graph do not reuse an event
   if (fork) {
@@ -398,18 +563,18 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
   }
   HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
   std::swap(c->c0a, c->c0b);
-  if (!a.nf1) return JP_OK;
+  if (!a.resident && !a.nf1) return JP_OK;
   if (!a.skipprep)  // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
-    HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, a.nf1, float(c->d.nqs01), dec(c, 0), st));
+    HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), st));
   if (fork) HCK(c, hipStreamWaitEvent(st, ev_join, 0));
-  HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+  HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));
   std::swap(a.f1a, a.f1b);
-  if (a.nf2)
+  if (fork)
     if (int r = level2_pair(c, G, st)) return r;
   if (a.skiprestr)
-    HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), st));
+    HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));
   else  // the second level-1 substep restricts into level 0
-    HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, a.nf1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
+    HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
                                   a.l1pcb, a.l1oct, float(c->d.nqi10), enc(c, 0), st));
   std::swap(a.f1a, a.f1b);
   return JP_OK;
@@ -537,6 +702,7 @@ int amr_counts(jp_ctx* c, jp_counts* o) {
 }
 
 int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n) {
+  if (int r = amr_sync_counts(c)) return r;
   if (which == JP_TABLE_SKIN0 || which == JP_TABLE_SKIN1)
     if (int r = ensure_skins(c)) return r;
   const AmrState& a = c->amr;

@@ -48,36 +48,45 @@ struct AmrGeo {
   int N0, cgx, l1gx, N1, N2;
 };
 
+// Tiles of a per-tile launch. count points at the device-resident tile count (null: the width itself,
+// one workgroup per tile); width is the number of workgroups. Launches recorded in a graph that a regrid
+// must leave valid read the count and take a width fixed when they are recorded; the kernels loop when
+// the count exceeds it.
+struct Tiles {
+  const int* count;
+  int width;
+};
+
 // ---- amr.hip ----
-hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const void* gh, const int* nb, int ntiles,
+hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const void* gh, const int* nb, Tiles t,
                          double omega, float enc, float dec, hipStream_t st);
 // fine step + restriction into the parent level (dense level 0 when to_dense)
 hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
-                                  const int* nb, int ntiles, double omega, float enc, float dec, void* cf, const int* ptile,
+                                  const int* nb, Tiles t, double omega, float enc, float dec, void* cf, const int* ptile,
                                   const int* poct, float nqi, float cenc, hipStream_t st);
 // Ghost records of level 1 (from the dense level 0 c0) and level 2 (from level 1 f1): prep values of
 // the parent box computed on the fly, trilinear interpolation of the slots whose source tile is absent.
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* nb1, const uint32_t* slots, int nf1, float nqs, float dec, hipStream_t st);
+                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, hipStream_t st);
 hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, int nf2, float nqs,
+                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
                        float dec, hipStream_t st);
 hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct,
-                       int nf1, float nqs, float dec0, float enc1, hipStream_t st);
+                       Tiles t, float nqs, float dec0, float enc1, hipStream_t st);
 hipError_t amr_seed_l2(int prec, const AmrGeo& G, const void* f1, void* f2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, int nf2, float nqs, float dec1, float enc2,
+                       const int* f1pos, const int* l1grid, Tiles t, float nqs, float dec1, float enc2,
                        hipStream_t st);
 hipError_t amr_migrate_l1(int prec, const AmrGeo& G, const void* oldf1, void* newf1, const void* c0,
-                          const int* l1pcb, const int* l1oct, const int* o_ref0, const int* o_fof0, int nf1, float nqs,
+                          const int* l1pcb, const int* l1oct, const int* o_ref0, const int* o_fof0, Tiles t, float nqs,
                           float dec0, float enc1, hipStream_t st);
 hipError_t amr_migrate_l2(int prec, const AmrGeo& G, const void* oldf2, void* newf2, const void* newf1,
                           const int* l2pl1, const int* l2oct, const int* f1pos, const int* o_l1grid, const int* o_ref1,
-                          const int* o_fof1, const int* l1grid, int nf2, float nqs, float dec1, float enc2,
+                          const int* o_fof1, const int* l1grid, Tiles t, float nqs, float dec1, float enc2,
                           hipStream_t st);
 hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const void* f1, const void* f2,
                                const int* ref1, const int* fof1, const int* l1grid, const float* dec3, void* F,
                                hipStream_t st);
-hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int ntiles, float dec, float* sen,
+hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen,
                       hipStream_t st);
 
 // ---- regrid.hip ----
@@ -85,34 +94,48 @@ hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, int n
 enum {
   CNT_NREF0 = 0, CNT_NREF1, CNT_SKIN0, CNT_SKIN1, CNT_NDEEP, CNT_OVERFLOW, CNT_TGT, CNT_NSEL0,
   CNT_DIFF0, CNT_DIFF1,  // a regrid changed the level-0 / level-1 marks
+  CNT_NF1, CNT_NF2,      // level-1 / level-2 tile counts the per-tile launches cover (at most the pools)
+  CNT_HW1, CNT_HW2,      // their high-water marks since the initial build
   CNT_N
+};
+static_assert(CNT_HW1 - CNT_NF1 == 2 && CNT_HW2 - CNT_NF2 == 2, "a scan's high-water slot follows its count slot by 2");
+// Items of a regrid launch: n from the host, or *dn when dn is given (n is then only the launch width's
+// hint, and the kernels loop over what exceeds it).
+struct Items {
+  int n;
+  const int* dn;
 };
 // thr = ascending-sorted[count - tgt] of sen (entries with mask != 0 when mask is given), tgt =
 // min(cap, max(1, int(frac * count))); writes count to cnt[count_slot] and thr to *thr.
-hipError_t rg_select(const float* sen, const int* mask, int n, double frac, int cap, float* thr, int* cnt,
+hipError_t rg_select(const float* sen, const int* mask, Items n, double frac, int cap, float* thr, int* cnt,
                      int count_slot, hipStream_t st);
 hipError_t rg_mark_thr(const float* sen, const float* thr, int* ref, int n, hipStream_t st);
-hipError_t rg_mark_ref1(const float* sen, const float* thr, const int* deep, const int* cnt, int* ref1, int nf1,
+hipError_t rg_mark_ref1(const float* sen, const float* thr, const int* deep, const int* cnt, int* ref1, Items nf1,
                         hipStream_t st);
 hipError_t rg_set(int* ref, int n, int mode, int cgx, const int* deep, hipStream_t st);  // 0 all, 1 slab, 2 deep
-hipError_t rg_deepmask(const int* f1pos, const int* l1grid, int l1gx, int nf1, int* deep, hipStream_t st);
-// exclusive scan of 0/1 flags into rank; total into cnt[slot]
-hipError_t rg_scan(const int* flags, int n, int* rank, int* cnt, int slot, hipStream_t st);
+hipError_t rg_deepmask(const int* f1pos, const int* l1grid, int l1gx, Items nf1, int* deep, hipStream_t st);
+// Exclusive scan of 0/1 flags into rank; total into cnt[slot]. With tiles_slot >= 0 also the tiles that
+// the total refines, 8 per flag, limited to a pool of cap_tiles: cnt[tiles_slot] = 8 min(total, cap_tiles / 8),
+// and their high-water mark in cnt[tiles_slot + 2].
+hipError_t rg_scan(const int* flags, Items n, int* rank, int* cnt, int slot, int tiles_slot, int cap_tiles,
+                   hipStream_t st);
 // ascending indices i with flags[i] != 0 into out; count into cnt[slot]
-hipError_t rg_compact(const int* flags, int n, int* out, int* cnt, int slot, hipStream_t st);
-hipError_t rg_scatter_l1(const int* ref, const int* rank, int cnb0, int cgx, int l1gx, int cap1, int* fof0, int* rl0,
+hipError_t rg_compact(const int* flags, Items n, int* out, int* cnt, int slot, hipStream_t st);
+// Block tables from the marks and their ranks; parents whose tiles exceed the pool raise the overflow
+// flag and, with clamp, are unmarked (the tables then describe the grid that fits).
+hipError_t rg_scatter_l1(int* ref, const int* rank, int cnb0, int cgx, int l1gx, int cap1, bool clamp, int* fof0, int* rl0,
                          int* l1pcb, int* l1oct, int* l1grid, int* f1pos, int* cnt, hipStream_t st);
-hipError_t rg_scatter_l2(const int* ref1, const int* rank, int nf1, int cap2, int* fof1, int* rl1, int* l2pl1,
+hipError_t rg_scatter_l2(int* ref1, const int* rank, Items nf1, int cap2, bool clamp, int* fof1, int* rl1, int* l2pl1,
                          int* l2oct, int* cnt, hipStream_t st);
 hipError_t rg_dilate_l0(const int* ref0, int cnb0, int cgx, int* flags, hipStream_t st);
-hipError_t rg_dilate_l1(const int* ref1, const int* f1pos, const int* l1grid, int nf1, int l1gx, int* flags,
+hipError_t rg_dilate_l1(const int* ref1, const int* f1pos, const int* l1grid, Items nf1, int l1gx, int* flags,
                         hipStream_t st);
-hipError_t rg_nbr_l1(const int* f1pos, const int* l1grid, int l1gx, int nf1, int* nb1, hipStream_t st);
+hipError_t rg_nbr_l1(const int* f1pos, const int* l1grid, int l1gx, Items nf1, int* nb1, hipStream_t st);
 hipError_t rg_nbr_l2(const int* l2pl1, const int* l2oct, const int* f1pos, const int* l1grid, const int* ref1,
-                     const int* fof1, int l1gx, int nf2, int* nb2, hipStream_t st);
-hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, int nf1, float* out, hipStream_t st);
+                     const int* fof1, int l1gx, Items nf2, int* nb2, hipStream_t st);
+hipError_t rg_gather_by_pos(const float* by_pos, const int* f1pos, Items nf1, float* out, hipStream_t st);
 // cnt[slot] = 1 if a[i] != b[i] for some i < n, else 0
-hipError_t rg_differs(const int* a, const int* b, int n, int* cnt, int slot, hipStream_t st);
+hipError_t rg_differs(const int* a, const int* b, Items n, int* cnt, int slot, hipStream_t st);
 // up to 5 copies src[k] -> dst[k] of n[k] ints in one launch
 hipError_t rg_copy_i32_multi(const int* const* src, int* const* dst, const long* n, int count, hipStream_t st);
 

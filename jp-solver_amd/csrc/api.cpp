@@ -57,10 +57,8 @@ double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 /
 int default_variant(int prec) { return prec == 0 ? 62 : (prec == 3 ? 12 : 6); }
 
 int drop_graph(jp_ctx* c) {
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
   return JP_OK;
 }
 
@@ -242,26 +240,46 @@ static int eager_steps(jp_ctx* c, long n) {
     int r = c->kind == 2 ? ens_step_once(c) : c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
     if (r) return r;
   }
-  if (n % 2) drop_graph(c);  // the buffers changed roles: a captured pair no longer applies
   return JP_OK;
 }
 
-// Captures two steps (the ping-pong buffers return to their roles) into the context's graph.
-static int ensure_graph(jp_ctx* c) {
-  if (c->gexec) return JP_OK;
+// The buffers a pair of steps starts from: after two steps every ping-pong pair is back in its role, so
+// a captured pair applies exactly when the current roles equal those it was captured with.
+static void graph_key(const jp_ctx* c, const void* key[3]) {
+  if (c->kind == 2) {
+    ens_graph_key(c, key);
+    return;
+  }
+  key[0] = c->c0a;
+  key[1] = c->kind == 1 ? c->amr.f1a : nullptr;
+  key[2] = c->kind == 1 ? c->amr.f2a : nullptr;
+}
+
+// The captured pair of steps for the current roles, captured on first use. A few pairs are kept: the
+// two parities of the dense buffers and, with device-resident regrids (which swap both fine pairs), of
+// the fine buffers.
+static int graph_for(jp_ctx* c, hipGraphExec_t* out) {
+  const void* key[3];
+  graph_key(c, key);
+  for (const auto& g : c->graphs)
+    if (g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2]) {
+      *out = g.exec;
+      return JP_OK;
+    }
+  if (c->graphs.size() >= 8) drop_graph(c);
   hipGraph_t g;
   HCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   const int rc = eager_steps(c, 2);
   hipError_t ec = hipStreamEndCapture(c->st, &g);
   if (rc) return rc;
   if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
-  ec = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipGraphExec_t exec = nullptr;
+  ec = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (ec != hipSuccess) {
-    c->gexec = nullptr;
-    return fail(c, JP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
-  }
-  HCK(c, hipGraphUpload(c->gexec, c->st));
+  if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
+  c->graphs.push_back({{key[0], key[1], key[2]}, exec});
+  HCK(c, hipGraphUpload(exec, c->st));
+  *out = exec;
   return JP_OK;
 }
 
@@ -271,15 +289,12 @@ int jp_step(jp_ctx* c, long n) {
   if (c->kind == 2)
     if (int r = ens_check(c)) return r;
   if (!c->use_graph) return eager_steps(c, n);
-  if (n >= 2) {
-    if (int r = ensure_graph(c)) return r;
-    for (long p = 0; p < n / 2; ++p) HCK(c, hipGraphLaunch(c->gexec, c->st));
+  for (long p = 0; p < n / 2; ++p) {
+    hipGraphExec_t g;
+    if (int r = graph_for(c, &g)) return r;
+    HCK(c, hipGraphLaunch(g, c->st));
   }
-  if (n % 2) {
-    drop_graph(c);  // an odd step swaps the buffers the graph was captured with
-    return eager_steps(c, 1);
-  }
-  return JP_OK;
+  return n % 2 ? eager_steps(c, 1) : JP_OK;
 }
 
 int jp_set_variant(jp_ctx* c, int variant) {
@@ -294,6 +309,10 @@ int jp_use_graph(jp_ctx* c, int on) {
   if (!c) return JP_ERR_ARG;
   c->use_graph = on != 0;
   if (!c->use_graph) drop_graph(c);
+  if (c->kind == 1) {
+    HCK(c, hipSetDevice(c->device));
+    return amr_set_resident(c, c->use_graph);
+  }
   return JP_OK;
 }
 
@@ -302,6 +321,8 @@ int jp_diag(jp_ctx* c, double out[3]) {
   if (c->kind == 2) return fail(c, JP_ERR_STATE, "diagnostics are per member (jp_ensemble_member)");
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_diag before jp_init");
   HCK(c, hipSetDevice(c->device));
+  if (c->kind == 1)  // a diagnostic point also reports a device-resident regrid that overflowed a pool
+    if (int r = amr_sync_counts(c)) return r;
   const int nf = c->kind == 1 ? c->amr.N2 : c->d.n;
   if (int r = ensure_diag_buffers(c, nf)) return r;
   if (c->kind == 1) {
@@ -363,6 +384,8 @@ int jp_get_derived(jp_ctx* c, jp_derived* d) {
 int jp_population_bytes(jp_ctx* c, int level, size_t* bytes) {
   if (!c || !bytes) return JP_ERR_ARG;
   if (c->kind == 2) return fail(c, JP_ERR_STATE, "populations are per member (jp_ensemble_member)");
+  if (c->kind == 1)
+    if (int r = amr_sync_counts(c)) return r;
   long tiles;
   if (level == 0) tiles = c->d.cells / MB3;
   else if (c->kind == 1 && level == 1) tiles = c->amr.nf1;
@@ -434,48 +457,81 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
   if (!c || !res || steps < 1 || warmup < 0) return JP_ERR_ARG;
   if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_bench before jp_init");
   HCK(c, hipSetDevice(c->device));
-  // src/tgv3d_amr.cu:839-864: a dynamic AMR run regrids every `adapt` steps, in warm-up and timed loop
+  // src/tgv3d_amr.cu:839-864: a dynamic AMR run regrids every `adapt` steps, in warm-up and timed loop.
+  // With graphs, an AMR context regrids on device-resident counts between replays, on the same schedule;
+  // an ensemble replays pairs of steps without regrids, as the reference's graph bench does.
   const long adapt = c->kind == 1 ? ((amr_dynamic(c) && c->amr.adapt > 0) ? c->amr.adapt : 0)
                                    : (c->kind == 2 ? ens_adapt(c) : 0);
-  const bool dyn = adapt > 0;
+  const bool graphs = use_graph != 0;
+  const bool regrids = adapt > 0 && !(graphs && c->kind == 2);
+  const bool prev_graph = c->use_graph;
+  if (graphs != prev_graph)
+    if (int r = jp_use_graph(c, graphs ? 1 : 0)) return r;
+  if (graphs && !regrids && steps % 2) steps += 1;  // whole pairs
   auto regrid = [&]() { return c->kind == 2 ? ens_regrid(c) : amr_regrid_with(c, nullptr, nullptr); };
-  for (long w = 0; w < warmup; ++w) {
-    if (dyn && w % adapt == 0)
-      if (int r = regrid()) return r;
-    if (int r = jp_step(c, 1)) return r;
-  }
+  // Runs `total` steps with a regrid at every multiple of adapt; per segment between regrids it calls
+  // seg(length) after the segment's steps are enqueued.
+  auto run = [&](long total, auto&& seg) -> int {
+    for (long s = 0; s < total;) {
+      if (regrids && s % adapt == 0)
+        if (int r = regrid()) return r;
+      const long next = regrids ? std::min(total, (s / adapt + 1) * adapt) : total;
+      if (int r = graphs ? jp_step(c, next - s) : eager_steps(c, next - s)) return r;
+      if (int r = seg(next - s)) return r;
+      s = next;
+    }
+    return JP_OK;
+  };
+  if (int r = run(warmup, [](long) { return JP_OK; })) return r;
   if (c->kind == 2)
     if (int r = ens_check(c)) return r;
   HCK(c, hipStreamSynchronize(c->st));
-  auto step_work = [&]() { return c->kind == 2 ? ens_work(c) : c->kind == 1 ? amr_work(c) : double(c->d.cells); };
+  if (c->kind == 1)
+    if (int r = amr_sync_counts(c)) return r;
+  // Work of a segment in cell updates: the level counts it ran with. After a device-resident regrid the
+  // host does not know them, so each segment's counts are copied to pinned memory behind its steps and
+  // summed after the timed loop.
+  const bool log_counts = c->kind == 1 && c->amr.resident;
+  std::vector<long> seg_len;
+  int* hlog = nullptr;
+  const long max_segs = regrids ? steps / adapt + 2 : 1;
+  if (log_counts) HCK(c, hipHostMalloc(reinterpret_cast<void**>(&hlog), size_t(max_segs) * 2 * sizeof(int), hipHostMallocDefault));
   hipEvent_t e0, e1;
   HCK(c, hipEventCreate(&e0));
   HCK(c, hipEventCreate(&e1));
   double work = 0;
-  if (use_graph) {
-    // two steps per graph so the ping-pong buffers return to their starting roles; no regrid inside
-    if (steps % 2) steps += 1;
-    if (int r = ensure_graph(c)) return r;
-    HCK(c, hipStreamSynchronize(c->st));
-    HCK(c, hipEventRecord(e0, c->st));
-    for (long r = 0; r < steps / 2; ++r) HCK(c, hipGraphLaunch(c->gexec, c->st));
-    HCK(c, hipEventRecord(e1, c->st));
-    work = step_work() * double(steps);
-  } else {
-    HCK(c, hipEventRecord(e0, c->st));
-    for (long s = 0; s < steps; ++s) {
-      if (dyn && s % adapt == 0)
-        if (int r = regrid()) return r;
-      if (int r = eager_steps(c, 1)) return r;
-      work += step_work();
+  HCK(c, hipEventRecord(e0, c->st));
+  int rc = run(steps, [&](long n) -> int {
+    if (log_counts) {
+      const hipError_t e = amr_log_counts(c, hlog + 2 * seg_len.size());
+      if (e != hipSuccess) return fail(c, JP_ERR_HIP, hipGetErrorString(e));
+    } else {
+      work += double(n) * (c->kind == 2 ? ens_work(c) : c->kind == 1 ? amr_work(c) : double(c->d.cells));
     }
-    HCK(c, hipEventRecord(e1, c->st));
+    seg_len.push_back(n);
+    return JP_OK;
+  });
+  if (rc == JP_OK) {
+    const hipError_t e = hipEventRecord(e1, c->st);
+    rc = e == hipSuccess ? JP_OK : fail(c, JP_ERR_HIP, hipGetErrorString(e));
   }
-  HCK(c, hipEventSynchronize(e1));
+  if (rc == JP_OK) {
+    const hipError_t e = hipEventSynchronize(e1);
+    rc = e == hipSuccess ? JP_OK : fail(c, JP_ERR_HIP, hipGetErrorString(e));
+  }
   float ms = 0;
-  HCK(c, hipEventElapsedTime(&ms, e0, e1));
+  if (rc == JP_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+  if (rc == JP_OK && log_counts)
+    for (size_t k = 0; k < seg_len.size(); ++k)
+      work += double(seg_len[k]) * double(c->amr.cnb0 + 2L * hlog[2 * k] + 4L * hlog[2 * k + 1]) * MB3;
+  if (hlog) (void)hipHostFree(hlog);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (c->kind == 1)
+    if (int r = amr_sync_counts(c)) return r;  // reports an overflowing resident regrid
+  if (graphs != prev_graph)
+    if (int r = jp_use_graph(c, prev_graph ? 1 : 0)) return r;
   res->steps = steps;
   res->ms_total = ms;
   res->ms_per_step = ms / double(steps);

@@ -25,6 +25,10 @@ struct AmrState {
   bool skins_stale = true;  // skin lists and their counts are rebuilt on demand after a regrid
   int cap2_alloc = 0;  // level-2 tiles the allocated pool holds
   int pool_gen = 0;    // bumped whenever a fine-level pool is reallocated (batched descriptors go stale)
+  bool resident_req = false;  // regrids on device-resident counts requested (graphs on, sensor-driven grid)
+  bool resident = false;      // ... and in effect: pools sized for them, every launch sized on the device
+  bool counts_stale = false;  // a device-resident regrid ran since the host last read the counts
+  int w1 = 8, w2 = 8;         // launch widths of the level-1 / level-2 launches while resident
   hipStream_t st2 = nullptr;  // side stream of the coarse step (first level-2 pair)
   hipEvent_t ev_fork[2] = {}, ev_join[2] = {};
   int evi = 0;  // f1b still has a pre-growth size (until the regrid's migration completes)
@@ -60,7 +64,13 @@ struct jp_ctx {
   float* host_out = nullptr;  // pinned host copy of them for the frame writer
   size_t host_out_bytes = 0;
   double* partial = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // Captured pairs of steps, keyed by the buffers they start from (the ping-pong roles), so that an odd
+  // step or a device-resident regrid selects another pair instead of recapturing.
+  struct Graph {
+    const void* key[3];
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
   std::vector<std::pair<void*, size_t>> allocs;
   size_t pool_bytes = 0;
   jp::AmrState amr;
@@ -88,6 +98,9 @@ int amr_create(const jp_amr_params* p, jp_ctx* c);
 int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
 int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos);
 int amr_counts(jp_ctx* c, jp_counts* o);
+int amr_sync_counts(jp_ctx* c);
+hipError_t amr_log_counts(jp_ctx* c, int* dst);  // level-1 and level-2 tile counts to pinned dst[2], async
+int amr_set_resident(jp_ctx* c, bool on);
 int amr_table(jp_ctx* c, int which, void* dst, long cap, long* n);
 double amr_work(const jp_ctx* c);
 bool amr_dynamic(const jp_ctx* c);
@@ -102,6 +115,7 @@ int ens_adapt(const jp_ctx* c);
 double ens_work(const jp_ctx* c);
 int ens_autotune(jp_ctx* c, int* out);
 void ens_destroy(jp_ctx* c);
+void ens_graph_key(const jp_ctx* c, const void* key[3]);  // the batch's buffer parities
 jp_ctx* ens_member(jp_ctx* c, int i);
 }  // namespace jp
 

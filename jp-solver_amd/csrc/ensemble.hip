@@ -36,12 +36,13 @@ __global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
+    const int lane = opaque_lane();
     if constexpr (kLevel == 1)
       fill_l1_tile<S>(cr<S>(d.c0[p]), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.nb1, d.slots, d.G, d.nqs01, d.dec[0], w.y,
-                      threadIdx.x, L);
+                      lane, L);
     else
       fill_l2_tile<S>(cr<S>(d.f1[p]), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.nb2, d.slots, d.G,
-                      d.nqs12, d.dec[1], w.y, threadIdx.x, L);
+                      d.nqs12, d.dec[1], w.y, lane, L);
   }
 }
 // items: (member, fine tile)
@@ -54,15 +55,16 @@ __global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
     const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
+    const int lane = opaque_lane();
     __syncthreads();  // the previous item's lanes are done with snb
-    if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(w.y) * kNbr + threadIdx.x];
+    if (lane < kNbr) snb[lane] = nb[size_t(w.y) * kNbr + lane];
     __syncthreads();
     if constexpr (kLevel == 1)
       fine_step_cell<S>(cr<S>(d.f1[p]), wr<S>(d.f1[p ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]), d.enc[1],
-                        d.dec[1], w.y, threadIdx.x);
+                        d.dec[1], w.y, lane);
     else
       fine_step_cell<S>(cr<S>(d.f2[p]), wr<S>(d.f2[p ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]), d.enc[2],
-                        d.dec[2], w.y, threadIdx.x);
+                        d.dec[2], w.y, lane);
   }
 }
 // Second substep of a fine level with the restriction into its parent fused in (the single-context
@@ -77,18 +79,19 @@ __global__ __launch_bounds__(512) void kb_fine_step_r(const MemberDesc* __restri
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
     const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
+    const int lane = opaque_lane();
     __syncthreads();  // the previous item's lanes are done with snb and the partial sums
-    if (threadIdx.x < kNbr) snb[threadIdx.x] = nb[size_t(w.y) * kNbr + threadIdx.x];
+    if (lane < kNbr) snb[lane] = nb[size_t(w.y) * kNbr + lane];
     __syncthreads();
     R kept[Q];
     if constexpr (kLevel == 1) {
       fine_step_cell<S, true>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]),
-                              d.enc[1], d.dec[1], w.y, threadIdx.x, kept);
-      restrict_tile<S, true>(kept, threadIdx.x, L, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
+                              d.enc[1], d.dec[1], w.y, lane, kept);
+      restrict_tile<S, true>(kept, lane, L, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
     } else {
       fine_step_cell<S, true>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]),
-                              d.enc[2], d.dec[2], w.y, threadIdx.x, kept);
-      restrict_tile<S, false>(kept, threadIdx.x, L, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
+                              d.enc[2], d.dec[2], w.y, lane, kept);
+      restrict_tile<S, false>(kept, lane, L, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
     }
   }
 }

@@ -143,6 +143,12 @@ int ens_check(jp_ctx* c) {
   return ok ? JP_OK : upload(c);
 }
 
+void ens_graph_key(const jp_ctx* c, const void* key[3]) {
+  const EnsState& e = *c->ens;
+  const int p[3] = {e.p0, e.p1, e.p2};
+  for (int l = 0; l < 3; ++l) key[l] = reinterpret_cast<const void*>(uintptr_t(1 + p[l]));
+}
+
 static void swap_all(EnsState& e, int level) {
   for (jp_ctx* m : e.m) {
     if (level == 0) std::swap(m->c0a, m->c0b);

@@ -91,6 +91,7 @@ int jp_write_vti(jp_ctx* c, const char* path) {
 int jp_write_boxes(jp_ctx* c, const char* path) {
   if (!c || !path) return JP_ERR_ARG;
   if (c->kind != 1) return jp::fail(c, JP_ERR_STATE, "level boxes exist for AMR contexts only");
+  if (int r = jp::amr_sync_counts(c)) return r;
   const jp::AmrState& a = c->amr;
   auto table = [&](int which, std::vector<int>& out, long len) -> int {
     out.resize(size_t(len));

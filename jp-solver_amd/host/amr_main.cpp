@@ -76,14 +76,19 @@ int main(int argc, char** argv) {
     jp_destroy(c);
     return 0;
   }
-  if (graph) jp_use_graph(c, 1);
+  // --graph (src/tgv3d_amr.cu:872-881): pairs of coarse steps replayed from captured graphs. The regrids
+  // run on device-resident counts, so a regrid needs no recapture; diagnostics and frames are taken
+  // between replays.
+  if (graph)
+    if (int rc = jp_use_graph(c, 1)) return die(c, rc);
+  const long sstep = graph ? 2 : 1;
   const long diagK = argi(argc, argv, "--diag-every", int(std::max(1L, steps / 160)));
   const int frames = argi(argc, argv, "--frames", 160);  // src/tgv3d_amr.cu:869
   const long frameK = vtk ? std::max(1L, steps / std::max(1, frames)) : 0;
   int fi = 0;
   double E0 = -1, peak_ens = -1, peak_ts = -1, prevE = 1e30, mass0 = -1, mass_drift = 0;
   bool mono = true;
-  for (long s = 0; s <= steps; ++s) {
+  for (long s = 0; s <= steps; s += sstep) {
     if (dynamic && s > 0 && s % p.adapt == 0)
       if (int rc = jp_regrid(c)) return die(c, rc);
     if (s % diagK == 0) {
@@ -111,7 +116,7 @@ int main(int argc, char** argv) {
       fi++;
     }
     if (s < steps)
-      if (int rc = jp_step(c, 1)) return die(c, rc);
+      if (int rc = jp_step(c, sstep)) return die(c, rc);
   }
   if (int rc = jp_sync(c)) return die(c, rc);
   jp_get_counts(c, &k);
