@@ -6,6 +6,9 @@ captured pairs of steps stay valid across regrids. The bar: the same run as the 
 counts, recapture after every regrid) bit for bit, in tables, counts and populations, and the oracle's
 tables and populations bit for bit; a count beyond a budget-limited pool is reported, never written.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -153,3 +156,18 @@ def test_resident_overflow_is_reported_not_written():
     with pytest.raises(jpabi.JPError) as err:
         g.step(2)
     assert err.value.code == jpabi.JP_ERR_STATE
+
+
+def test_amr_driver_graph_loop_matches_eager_loop():
+    # the production loop of ./amr --graph (src/tgv3d_amr.cu:872-881): pairs of coarse steps from graphs,
+    # regrids on device-resident counts between them; the diagnostics must equal the eager loop's
+    build = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jp-solver_amd", "build")
+    args = [os.path.join(build, "amr"), "--n", "24", "--re", "1600", "--l1", "sensor", "--l2", "sensor", "--frac1", "0.3",
+            "--frac2", "0.3", "--adapt", "4", "--tstar", "0.21", "--diag-every", "2", "--maxgb", "4"]
+    out = []
+    for graph in ("0", "1"):
+        r = subprocess.run(args + ["--graph", graph], capture_output=True, text=True, timeout=120)
+        assert r.returncode in (0, 1), r.stdout + r.stderr
+        lines = [x for x in r.stdout.splitlines() if x.startswith("  t*=") or "enstrophy peak" in x]
+        out.append(lines)
+    assert len(out[0]) == 10 and out[0] == out[1]
