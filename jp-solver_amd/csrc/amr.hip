@@ -6,6 +6,7 @@
 // the tile's 27 same-level neighbours (precomputed at regrid) once in LDS, then pulls each direction
 // from the own tile, a neighbour tile, or the tile's compact ghost record.
 #include "amr_bodies.hpp"
+#include "dense_bodies.hpp"
 
 namespace jp {
 
@@ -18,7 +19,7 @@ __global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __rest
   for_tiles<kLoop>(dn, [=](int b, int lane) {
     if (lane < kNbr) snb[lane] = nb[size_t(b) * kNbr + lane];
     __syncthreads();
-    fine_step_cell<S>(fa, fb, gh, snb, om, enc, dec, b, lane);
+    fine_step_cell<S>(fa, fb, gh + size_t(b) * kGhostSlots, snb, om, enc, dec, b, lane);
   });
 }
 // Last fine substep with the restriction into the parent level fused in (ptile/poct: the parent tile
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __re
     if (lane < kNbr) snb[lane] = nb[size_t(b) * kNbr + lane];
     __syncthreads();
     typename S::real kept[Q];
-    fine_step_cell<S, true>(fa, fb, gh, snb, om, enc, dec, b, lane, kept);
+    fine_step_cell<S, true>(fa, fb, gh + size_t(b) * kGhostSlots, snb, om, enc, dec, b, lane, kept);
     restrict_tile<S, kToDense>(kept, lane, L, cf, ptile[b], poct[b], G, nqi, cenc);
   });
 }
@@ -60,6 +61,94 @@ __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restri
     fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, b, lane, L);
   });
 }
+// Level-2 ghost fill fused with the first level-2 substep of the pair that reads it: the workgroup fills
+// its tile's ghost record (written to gh2 for the pair's second substep, and kept in LDS) and steps the tile
+// with the ghost values read from LDS, one launch and one pass over the tile's neighbourhood less.
+template <class S, bool kLoop>
+__global__ __launch_bounds__(512) void k_fill_step_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
+                                                      const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
+                                                      const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                                      const int* __restrict__ nb2, const uint32_t* __restrict__ slots,
+                                                      const int* __restrict__ dn, AmrGeo G, float nqs, float dec1,
+                                                      const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                                      typename S::real om, float enc2, float dec2) {
+  using R = typename S::real;
+  __shared__ FillLds<R> L;
+  __shared__ R ghost[kGhostSlots];
+  for_tiles<kLoop, false>(dn, [=](int b, int lane) {
+    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec1, b, lane, L, ghost);
+    __syncthreads();
+    fine_step_cell<S>(fa, fb, ghost, L.nb, om, enc2, dec2, b, lane);
+  });
+}
+
+// The first launches of a coarse step when level 2 exists. The first level-2 pair touches only levels 1
+// and 2 while the level-0 step and the level-1 fill touch only level 0 and the level-1 ghost record, so
+// each launch below carries two independent parts in disjoint workgroup ranges (the first part first in
+// dispatch order): no second stream, no cross-stream wait, and the parts share the GPU. Each part starts
+// at a multiple of 8 workgroups so that workgroup indices keep their XCD.
+//  - k_dense_fill_step_l2: the level-0 step (one cell per lane, nd workgroups) and the fused level-2 fill
+//    and first substep of the pair;
+//  - k_step_r_l2_fill_l1: the pair's second level-2 substep with the restriction into level 1 (w2
+//    workgroups) and the level-1 ghost fill from the new level 0.
+template <class S, bool kLoop>
+__global__ __launch_bounds__(512) void k_dense_fill_step_l2(const typename S::raw* __restrict__ c0a,
+                                                            typename S::raw* __restrict__ c0b, Dense g0,
+                                                            typename S::real om0, float enc0, float dec0, int nd,
+                                                            const typename S::raw* __restrict__ f1,
+                                                            typename S::real* __restrict__ gh2, const int* __restrict__ l2pl1,
+                                                            const int* __restrict__ l2oct, const int* __restrict__ f1pos,
+                                                            const int* __restrict__ l1grid, const int* __restrict__ nb2,
+                                                            const uint32_t* __restrict__ slots, int n2,
+                                                            const int* __restrict__ dn2, AmrGeo G, float nqs, float dec1,
+                                                            const typename S::raw* __restrict__ fa,
+                                                            typename S::raw* __restrict__ fb, typename S::real om2,
+                                                            float enc2, float dec2) {
+  using R = typename S::real;
+  __shared__ FillLds<R> L;
+  __shared__ R ghost[kGhostSlots];
+  if (int(blockIdx.x) < nd) {
+    const uint32_t cell = blockIdx.x * 512 + threadIdx.x;
+    if (cell < g0.N * g0.N * g0.N) dense_step_cell<S, false, false>(c0a, c0b, g0, om0, enc0, dec0, cell);
+    return;
+  }
+  for_tiles_in<kLoop, false>(n2, dn2, int(blockIdx.x) - nd, int(gridDim.x) - nd, [=](int b, int lane) {
+    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec1, b, lane, L, ghost);
+    __syncthreads();
+    fine_step_cell<S>(fa, fb, ghost, L.nb, om2, enc2, dec2, b, lane);
+  });
+}
+template <class S, bool kLoop>
+__global__ __launch_bounds__(512) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                                           const typename S::real* __restrict__ gh2,
+                                                           const int* __restrict__ nb2, int n2, const int* __restrict__ dn2,
+                                                           int w2, typename S::real om2, float enc2, float dec2,
+                                                           typename S::raw* __restrict__ f1, const int* __restrict__ l2pl1,
+                                                           const int* __restrict__ l2oct, AmrGeo G, float nqi21, float enc1,
+                                                           const typename S::raw* __restrict__ c0,
+                                                           typename S::real* __restrict__ gh1, const int* __restrict__ l1pcb,
+                                                           const int* __restrict__ l1oct, const int* __restrict__ nb1,
+                                                           const uint32_t* __restrict__ slots, int n1,
+                                                           const int* __restrict__ dn1, float nqs01, float dec0) {
+  using R = typename S::real;
+  __shared__ int snb[kNbr];
+  __shared__ RestrictLds<R> RL;
+  __shared__ FillLds<R> L;
+  if (int(blockIdx.x) < w2) {
+    for_tiles_in<kLoop>(n2, dn2, blockIdx.x, w2, [=](int b, int lane) {
+      if (lane < kNbr) snb[lane] = nb2[size_t(b) * kNbr + lane];
+      __syncthreads();
+      R kept[Q];
+      fine_step_cell<S, true>(fa, fb, gh2 + size_t(b) * kGhostSlots, snb, om2, enc2, dec2, b, lane, kept);
+      restrict_tile<S, false>(kept, lane, RL, f1, l2pl1[b], l2oct[b], G, nqi21, enc1);
+    });
+    return;
+  }
+  for_tiles_in<kLoop, false>(n1, dn1, int(blockIdx.x) - w2, int(gridDim.x) - w2, [=](int b, int lane) {
+    fill_l1_tile<S, 512>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs01, dec0, b, lane, L);
+  });
+}
+
 // prolong_L0 (src/tgv3d_amr.cu:70-91): trilinear over 8 periodic coarse cells, each corner's
 // non-equilibrium regularised and rescaled by nqs.
 template <class S> __device__ inline void prolong_l0(const typename S::raw* __restrict__ c0, const AmrGeo& G, int gfi,
@@ -321,6 +410,47 @@ hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, con
   if (t.width <= 0) return hipSuccess;
   JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_l2<S, kLoop>), dim3(t.width), dim3(256), 0, st, RAW(f1), WREAL(gh2), l2pl1, l2oct,
                                        f1pos, l1grid, nb2, slots, t.count, G, nqs, dec)));
+  return hipGetLastError();
+}
+
+hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+                            const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
+                            float dec1, const void* fa, void* fb, double omega, float enc2, float dec2, hipStream_t st) {
+  if (t.width <= 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_step_l2<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f1),
+                                                   WREAL(gh2), l2pl1, l2oct, f1pos, l1grid, nb2, slots, t.count, G, nqs, dec1,
+                                                   RAW(fa), WRAW(fb), typename S::real(omega), enc2, dec2)));
+  return hipGetLastError();
+}
+
+// Part width: the launch width of a Tiles rounded up to a multiple of 8 (one-tile form: the count).
+static inline int part8(Tiles t) { return t.width > 0 ? (t.width + 7) / 8 * 8 : 0; }
+
+hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, double om0, float enc0,
+                                  float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+                                  const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t2,
+                                  float nqs12, float dec1, const void* f2a, void* f2b, double om2, float enc2, float dec2,
+                                  hipStream_t st) {
+  const long cells = long(G.g0.N) * G.g0.N * G.g0.N;
+  const int nd = int((cells / 512 + 7) / 8 * 8), w2 = part8(t2);
+  JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_dense_fill_step_l2<S, kLoop>), dim3(nd + w2), dim3(512), 0, st,
+                                                    RAW(c0a), WRAW(c0b), G.g0, typename S::real(om0), enc0, dec0, nd,
+                                                    RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos, l1grid, nb2, slots, t2.width,
+                                                    t2.count, G, nqs12, dec1, RAW(f2a), WRAW(f2b),
+                                                    typename S::real(om2), enc2, dec2)));
+  return hipGetLastError();
+}
+hipError_t amr_step_r_l2_fill_l1(int prec, const AmrGeo& G, const void* f2a, void* f2b, const void* gh2, const int* nb2,
+                                 Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1, const int* l2oct,
+                                 float nqi21, float enc1, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
+                                 const int* nb1, const uint32_t* slots, Tiles t1, float nqs01, float dec0, hipStream_t st) {
+  const int w2 = part8(t2), w1 = part8(t1);
+  if (w1 + w2 == 0) return hipSuccess;
+  JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_step_r_l2_fill_l1<S, kLoop>), dim3(w2 + w1), dim3(512), 0, st,
+                                                    RAW(f2a), WRAW(f2b), REAL(gh2), nb2, t2.width, t2.count, w2,
+                                                    typename S::real(om2), enc2, dec2, WRAW(f1), l2pl1, l2oct, G, nqi21,
+                                                    enc1, RAW(c0), WREAL(gh1), l1pcb, l1oct, nb1, slots, t1.width,
+                                                    t1.count, nqs01, dec0)));
   return hipGetLastError();
 }
 

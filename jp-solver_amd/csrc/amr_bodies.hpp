@@ -59,17 +59,26 @@ __device__ __forceinline__ int opaque_lane() {
 // barrier precedes every index but the first, so that a body may rewrite the LDS the previous one used.
 // body(tile, lane) takes the lane index from the loop, in the looping form from opaque_lane(). The loop
 // still costs registers, which is why the one-tile form is a separate instantiation.
-template <bool kLoop, bool kSync = true, class F> __device__ __forceinline__ void for_tiles(const int* __restrict__ dn, F&& body) {
+// for_tiles_in: the same over a part of a launch, workgroups wg in [0, width) of it (width a multiple of 8
+// and the part starting at a multiple of 8, so that wg keeps its XCD); the one-tile form then skips tiles
+// from n up (width is n rounded up to a multiple of 8).
+template <bool kLoop, bool kSync = true, class F>
+__device__ __forceinline__ void for_tiles_in(int n, const int* __restrict__ dn, int wg, int width, F&& body) {
   if constexpr (!kLoop) {
-    body(xcd_tile(blockIdx.x, gridDim.x), int(threadIdx.x));
+    const int b = xcd_tile(wg, width);
+    if (b < n) body(b, int(threadIdx.x));
   } else {
-    const int n = *dn;
-    for (int v = blockIdx.x; v < n; v += gridDim.x) {
+    const int nd = *dn;
+    for (int v = wg; v < nd; v += width) {
       if constexpr (kSync)
-        if (v != int(blockIdx.x)) __syncthreads();
-      body(xcd_tile(v, n), opaque_lane());
+        if (v != wg) __syncthreads();
+      body(xcd_tile(v, nd), opaque_lane());
     }
   }
+}
+template <bool kLoop, bool kSync = true, class F> __device__ __forceinline__ void for_tiles(const int* __restrict__ dn, F&& body) {
+  if constexpr (!kLoop) body(xcd_tile(blockIdx.x, gridDim.x), int(threadIdx.x));
+  else for_tiles_in<kLoop, kSync>(0, dn, blockIdx.x, gridDim.x, body);
 }
 
 template <class S> __device__ __forceinline__ void load_tile(const typename S::raw* __restrict__ f, int b, int cell, float dec,
@@ -175,14 +184,14 @@ __device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q],
 // Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
 // 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
 // to this level); missing sources come from the ghost record.
-// snb: the tile's neighbour entries, staged in LDS by the caller.
+// snb: the tile's neighbour entries, staged in LDS by the caller; g: the tile's ghost record (in HBM, or
+// in LDS where the caller has just computed it).
 template <class S, bool kKeep = false>
 __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
-                                      const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
+                                      const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
   using R = typename S::real;
   const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
-  const R* __restrict__ g = gh + size_t(b) * kGhostSlots;
   R fin[Q];
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -205,10 +214,10 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
 // The same without keeping the stored values.
 template <class S>
 __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
-                                      const typename S::real* __restrict__ gh, const int* snb, typename S::real om,
+                                      const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell) {
   typename S::real none[Q];
-  fine_step_cell<S, false>(fa, fb, gh, snb, om, enc, dec, b, cell, none);
+  fine_step_cell<S, false>(fa, fb, g, snb, om, enc, dec, b, cell, none);
 }
 
 // prep = feq + nqs * regularize(f - feq) of one cell, in place (src/tgv3d_amr.cu:160-171).
@@ -236,7 +245,7 @@ template <class R> __device__ __forceinline__ void prep_values(R* v, float nqs) 
 // whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped, and only the
 // box cells that some remaining slot reads are staged (a tile with all 26 neighbours stages nothing).
 constexpr int kBox = kFillBox, kBox3 = kBox * kBox * kBox;
-constexpr int kFillThreads = 256, kSlotsPerLane = (kGhostSlots + kFillThreads - 1) / kFillThreads;
+constexpr int kFillThreads = 256;
 template <class R> struct FillLds {
   R box[Q * kBox3];
   int nb[kNbr];
@@ -260,13 +269,14 @@ __device__ __forceinline__ float corner_weight(uint32_t e, int c) {
   return ((c & 1) ? tx : 1 - tx) * ((c & 2) ? ty : 1 - ty) * ((c & 4) ? tz : 1 - tz);
 }
 
-// Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists).
-template <class S>
+// Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists); kThreads lanes.
+template <class S, int kThreads = kFillThreads>
 __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                     const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                     const int* __restrict__ nb1, const uint32_t* __restrict__ slots, const AmrGeo& G,
                                     float nqs, float dec, int fb, int tid, FillLds<typename S::real>& L) {
   using R = typename S::real;
+  constexpr int kSlotsPerLane = (kGhostSlots + kThreads - 1) / kThreads;
   const int cb = l1pcb[fb], oct = l1oct[fb];
   const int cbx = cb % G.cgx, cby = (cb / G.cgx) % G.cgx, cbz = cb / (G.cgx * G.cgx);
   const int ox = cbx * 2 * MB + (oct & 1) * MB, oy = cby * 2 * MB + ((oct >> 1) & 1) * MB, oz = cbz * 2 * MB + (oct >> 2) * MB;
@@ -274,13 +284,13 @@ __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__
   uint32_t ent[kSlotsPerLane];  // this lane's slot entries, loaded ahead of the staging (0: none)
 #pragma unroll
   for (int k = 0; k < kSlotsPerLane; ++k) {
-    const int s = tid + k * kFillThreads;
+    const int s = tid + k * kThreads;
     ent[k] = s < kGhostSlots ? slots[s] : 0u;
   }
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb1[size_t(fb) * kNbr + tid];
   __syncthreads();
-  for (int c = tid; c < kBox3; c += kFillThreads) {
+  for (int c = tid; c < kBox3; c += kThreads) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
     R v[Q];
@@ -298,19 +308,22 @@ __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__
     R out = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) out += R(corner_weight(e, c)) * bq[((c >> 2) * kBox + ((c >> 1) & 1)) * kBox + (c & 1)];
-    gh1[size_t(fb) * kGhostSlots + tid + k * kFillThreads] = out;
+    gh1[size_t(fb) * kGhostSlots + tid + k * kThreads] = out;
   }
 }
 
 // Level 2 from the scattered level 1: box cells without a level-1 tile are dropped and the remaining
-// weights renormalised (src/tgv3d_amr.cu:133-144, 187-202).
-template <class S>
+// weights renormalised (src/tgv3d_amr.cu:133-144, 187-202). kThreads lanes; lg (optional): an LDS copy
+// of the tile's ghost record for a step fused behind the fill.
+template <class S, int kThreads = kFillThreads>
 __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                     const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                     const int* __restrict__ f1pos, const int* __restrict__ l1grid,
                                     const int* __restrict__ nb2, const uint32_t* __restrict__ slots, const AmrGeo& G,
-                                    float nqs, float dec, int b, int tid, FillLds<typename S::real>& L) {
+                                    float nqs, float dec, int b, int tid, FillLds<typename S::real>& L,
+                                    typename S::real* lg = nullptr) {
   using R = typename S::real;
+  constexpr int kSlotsPerLane = (kGhostSlots + kThreads - 1) / kThreads;
   const int pl1 = l2pl1[b], oct = l2oct[b], p = f1pos[pl1];
   const int px = p % G.l1gx, py = (p / G.l1gx) % G.l1gx, pz = p / (G.l1gx * G.l1gx);
   const int ox = px * 2 * MB + (oct & 1) * MB, oy = py * 2 * MB + ((oct >> 1) & 1) * MB, oz = pz * 2 * MB + (oct >> 2) * MB;
@@ -318,13 +331,13 @@ __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__
   uint32_t ent[kSlotsPerLane];  // this lane's slot entries, loaded ahead of the staging (0: none)
 #pragma unroll
   for (int k = 0; k < kSlotsPerLane; ++k) {
-    const int s = tid + k * kFillThreads;
+    const int s = tid + k * kThreads;
     ent[k] = s < kGhostSlots ? slots[s] : 0u;
   }
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb2[size_t(b) * kNbr + tid];
   __syncthreads();
-  for (int c = tid; c < kBox3; c += kFillThreads) {
+  for (int c = tid; c < kBox3; c += kThreads) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     L.ok[c] = 0;
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
@@ -356,7 +369,8 @@ __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__
       out += R(wgt) * bq[o];
     }
     if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
-    gh2[size_t(b) * kGhostSlots + tid + k * kFillThreads] = out;
+    gh2[size_t(b) * kGhostSlots + tid + k * kThreads] = out;
+    if (lg) lg[tid + k * kThreads] = out;
   }
 }
 

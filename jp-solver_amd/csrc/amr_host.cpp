@@ -526,10 +526,11 @@ static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
   AmrState& a = c->amr;
   const int p = c->prec;
   const Tiles t2 = tiles(c, 2);
-  if (!a.skipprep)
-    HCK(c, amr_fill_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12),
-                       dec(c, 1), st));
-  HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+  if (a.skipprep)
+    HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
+  else  // fill and first substep in one launch
+    HCK(c, amr_fill_step_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12),
+                            dec(c, 1), a.f2a, a.f2b, c->d.omega2, enc(c, 2), dec(c, 2), st));
   std::swap(a.f2a, a.f2b);
   if (a.skiprestr)
     HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
@@ -542,34 +543,49 @@ static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
 
 // One coarse step (src/tgv3d_amr.cu:821-837). The first level-2 pair reads and writes only levels 1 and
 // 2 (its restriction touches the interior of refined level-1 tiles), while the level-0 step and the
-// level-1 fill touch only level 0 and the level-1 ghost record: the pair runs on a second stream,
-// overlapping them, and the first level-1 substep waits for both. Every buffer is written by at most one
-// of the two branches and read by the other only after the join, so the results are those of the
-// sequential order. With device-resident regrids the host does not know whether a level is empty, so
-// every launch is made (an empty level's workgroups read a zero count and return).
+// level-1 fill touch only level 0 and the level-1 ghost record, so the two run concurrently: as two
+// launches that each carry one part of both (the level-0 step with the pair's fused fill and first
+// substep, then the pair's restricting substep with the level-1 fill; see amr.hip), or, when the
+// profiling switches split the pair, on a second stream that the first level-1 substep waits for. Every
+// buffer is written by at most one of the two and read by the other only after both, so the results are
+// those of the sequential order. With device-resident regrids the host does not know whether a level is
+// empty, so every launch is made (an empty level's workgroups read a zero count and return).
 int amr_base_step(jp_ctx* c, hipStream_t st) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   const int p = c->prec;
-  const bool fork = a.resident || a.nf2 > 0;
-  const Tiles t1 = tiles(c, 1);
-  hipEvent_t ev_fork = a.ev_fork[a.evi], ev_join = a.ev_join[a.evi];
-  a.evi ^= 1;  // two event pairs: consecutive steps in one captured graph do not reuse an event
-  if (fork) {
-    HCK(c, hipEventRecord(ev_fork, st));
-    HCK(c, hipStreamWaitEvent(a.st2, ev_fork, 0));
-    if (int r = level2_pair(c, G, a.st2)) return r;
-    HCK(c, hipEventRecord(ev_join, a.st2));
+  const bool l2 = a.resident || a.nf2 > 0;
+  const Tiles t1 = tiles(c, 1), t2 = tiles(c, 2);
+  if (l2 && !a.skipprep && !a.skiprestr) {
+    HCK(c, amr_dense_fill_step_l2(p, G, c->c0a, c->c0b, c->d.omega0, enc(c, 0), dec(c, 0), a.f1a, a.gh2, a.l2pl1, a.l2oct,
+                                  a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12), dec(c, 1), a.f2a, a.f2b,
+                                  c->d.omega2, enc(c, 2), dec(c, 2), st));
+    std::swap(c->c0a, c->c0b);
+    std::swap(a.f2a, a.f2b);
+    // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
+    HCK(c, amr_step_r_l2_fill_l1(p, G, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a, a.l2pl1,
+                                 a.l2oct, float(c->d.nqi21), enc(c, 1), c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1,
+                                 float(c->d.nqs01), dec(c, 0), st));
+    std::swap(a.f2a, a.f2b);
+  } else {
+    hipEvent_t ev_fork = a.ev_fork[a.evi], ev_join = a.ev_join[a.evi];
+    a.evi ^= 1;  // two event pairs: consecutive steps in one captured graph do not reuse an event
+    if (l2) {
+      HCK(c, hipEventRecord(ev_fork, st));
+      HCK(c, hipStreamWaitEvent(a.st2, ev_fork, 0));
+      if (int r = level2_pair(c, G, a.st2)) return r;
+      HCK(c, hipEventRecord(ev_join, a.st2));
+    }
+    HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
+    std::swap(c->c0a, c->c0b);
+    if (!a.resident && !a.nf1) return JP_OK;
+    if (!a.skipprep)
+      HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), st));
+    if (l2) HCK(c, hipStreamWaitEvent(st, ev_join, 0));
   }
-  HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
-  std::swap(c->c0a, c->c0b);
-  if (!a.resident && !a.nf1) return JP_OK;
-  if (!a.skipprep)  // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
-    HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), st));
-  if (fork) HCK(c, hipStreamWaitEvent(st, ev_join, 0));
   HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));
   std::swap(a.f1a, a.f1b);
-  if (fork)
+  if (l2)
     if (int r = level2_pair(c, G, st)) return r;
   if (a.skiprestr)
     HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));

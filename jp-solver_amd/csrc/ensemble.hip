@@ -60,10 +60,10 @@ __global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict
     if (lane < kNbr) snb[lane] = nb[size_t(w.y) * kNbr + lane];
     __syncthreads();
     if constexpr (kLevel == 1)
-      fine_step_cell<S>(cr<S>(d.f1[p]), wr<S>(d.f1[p ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]), d.enc[1],
+      fine_step_cell<S>(cr<S>(d.f1[p]), wr<S>(d.f1[p ^ 1]), static_cast<const R*>(d.gh1) + size_t(w.y) * kGhostSlots, snb, R(d.om[1]), d.enc[1],
                         d.dec[1], w.y, lane);
     else
-      fine_step_cell<S>(cr<S>(d.f2[p]), wr<S>(d.f2[p ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]), d.enc[2],
+      fine_step_cell<S>(cr<S>(d.f2[p]), wr<S>(d.f2[p ^ 1]), static_cast<const R*>(d.gh2) + size_t(w.y) * kGhostSlots, snb, R(d.om[2]), d.enc[2],
                         d.dec[2], w.y, lane);
   }
 }
@@ -85,11 +85,11 @@ __global__ __launch_bounds__(512) void kb_fine_step_r(const MemberDesc* __restri
     __syncthreads();
     R kept[Q];
     if constexpr (kLevel == 1) {
-      fine_step_cell<S, true>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), snb, R(d.om[1]),
+      fine_step_cell<S, true>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1) + size_t(w.y) * kGhostSlots, snb, R(d.om[1]),
                               d.enc[1], d.dec[1], w.y, lane, kept);
       restrict_tile<S, true>(kept, lane, L, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
     } else {
-      fine_step_cell<S, true>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), snb, R(d.om[2]),
+      fine_step_cell<S, true>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2) + size_t(w.y) * kGhostSlots, snb, R(d.om[2]),
                               d.enc[2], d.dec[2], w.y, lane, kept);
       restrict_tile<S, false>(kept, lane, L, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
     }
