@@ -28,10 +28,10 @@ static AmrGeo geo(const jp_ctx* c) {
 static size_t real_bytes(int prec) { return prec == JP_PREC_FP64 ? 8 : 4; }
 static float enc(const jp_ctx* c, int l) { return float(c->d.scale[l]); }
 static float dec(const jp_ctx* c, int l) { return float(1.0 / c->d.scale[l]); }
-// Launch width of the device-counted launches: the last count the host knew with 1/8 headroom for the
-// counts of later regrids, in multiples of 8 (workgroups beyond the live count return at once; a count
-// beyond the width gives some workgroups a second tile).
-static int width8(long n) { return int(std::max(8L, (n + n / 8 + 7) / 8 * 8)); }
+// Launch width of the device-counted launches: the last count the host knew, in multiples of 8
+// (workgroups beyond the live count return at once, at a cost of a few microseconds per launch when
+// there are many; a count beyond the width gives some workgroups a second tile).
+static int width8(long n) { return int(std::max(8L, (n + 7) / 8 * 8)); }
 // A count beyond a pool leaves the grid incomplete: the context needs a new initial build before it
 // steps again.
 static int capacity_fail(jp_ctx* c, const std::string& msg) {
@@ -45,13 +45,13 @@ static int capacity_fail(jp_ctx* c, const std::string& msg) {
 // later regrid adds beyond it).
 static Tiles tiles(const jp_ctx* c, int level) {
   const AmrState& a = c->amr;
-  if (!a.resident) return {nullptr, level == 1 ? a.nf1 : a.nf2};
+  if (!a.resident && !a.dev_counts) return {nullptr, level == 1 ? a.nf1 : a.nf2};
   return {a.dcnt + (level == 1 ? CNT_NF1 : CNT_NF2), level == 1 ? a.w1 : a.w2};
 }
 static Items items(const jp_ctx* c, int level) {
   const AmrState& a = c->amr;
   const int* dn = a.dcnt + (level == 1 ? CNT_NF1 : CNT_NF2);
-  return {a.resident ? (level == 1 ? a.w1 : a.w2) : (level == 1 ? a.nf1 : a.nf2), dn};
+  return {(a.resident || a.dev_counts) ? (level == 1 ? a.w1 : a.w2) : (level == 1 ? a.nf1 : a.nf2), dn};
 }
 static Tiles tiles0(const jp_ctx* c) { return {nullptr, c->amr.cnb0}; }
 
@@ -86,10 +86,9 @@ static int fill(jp_ctx* c, int* p, int v, size_t n) {
 }
 // Host copies of the counts after device-resident regrids, read when a caller needs them. The overflow
 // flag is sticky: a resident regrid whose counts exceeded a pool left the grid incomplete.
-int amr_sync_counts(jp_ctx* c) {
+// The host copies from hcnt, just read.
+static int take_counts(jp_ctx* c) {
   AmrState& a = c->amr;
-  if (!a.counts_stale) return JP_OK;
-  if (int r = read_counts(c)) return r;
   a.counts_stale = false;
   a.nref0 = a.hcnt[CNT_NREF0];
   a.nf1 = a.hcnt[CNT_NF1];
@@ -104,6 +103,11 @@ int amr_sync_counts(jp_ctx* c) {
                                         std::to_string(8L * a.nref0) + "/" + std::to_string(a.cap1) + ", level-2 " +
                                         std::to_string(8L * a.nref1) + "/" + std::to_string(a.cap2) + "); raise --maxgb");
   return JP_OK;
+}
+int amr_sync_counts(jp_ctx* c) {
+  if (!c->amr.counts_stale) return JP_OK;
+  if (int r = read_counts(c)) return r;
+  return take_counts(c);
 }
 
 static bool pool_fits(const AmrState& a, long c1, long c2) {
@@ -343,10 +347,11 @@ static int snapshot_tables(jp_ctx* c) {
 }
 
 // The regrid on device-resident counts: the same pipeline with every level-1 and level-2 launch sized
-// on the device, no count read back and no pool growth (the pools hold the largest counts a regrid can
-// produce, see make_resident). Both fine levels always migrate, so each regrid swaps both buffer pairs
-// and the captured steps of either parity stay valid.
-static int regrid_resident(jp_ctx* c) {
+// on the device, no count read back and no pool growth. Both fine levels always migrate, so each regrid
+// swaps both buffer pairs and the captured steps of either parity stay valid. With clamp (regrids of a
+// resident context, whose pools hold the largest counts the budget allows, see apply_resident) a parent
+// beyond a pool is left unrefined; without it the tables keep such parents for the eager path to redo.
+static int regrid_device(jp_ctx* c, bool clamp) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   if (int r = snapshot_tables(c)) return r;
@@ -354,7 +359,7 @@ static int regrid_resident(jp_ctx* c) {
   HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
   HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
   HCK(c, rg_scan(a.ref0, {a.cnb0, nullptr}, a.rank, a.dcnt, CNT_NREF0, CNT_NF1, a.cap1, c->st));
-  if (int r = scatter_l1(c, true)) return r;
+  if (int r = scatter_l1(c, clamp)) return r;
   HCK(c, amr_migrate_l1(c->prec, G, a.f1a, a.f1b, c->c0a, a.l1pcb, a.l1oct, a.o_ref0, a.o_fof0, tiles(c, 1),
                         float(c->d.nqs01), dec(c, 0), enc(c, 1), c->st));
   std::swap(a.f1a, a.f1b);
@@ -363,7 +368,7 @@ static int regrid_resident(jp_ctx* c) {
   if (int r = fill(c, a.ref1, 0, size_t(a.cap1))) return r;
   HCK(c, rg_mark_ref1(a.sen1, a.thr, a.deep, a.dcnt, a.ref1, items(c, 1), c->st));
   HCK(c, rg_scan(a.ref1, items(c, 1), a.rank, a.dcnt, CNT_NREF1, CNT_NF2, a.cap2, c->st));
-  if (int r = scatter_l2(c, true)) return r;
+  if (int r = scatter_l2(c, clamp)) return r;
   HCK(c, amr_migrate_l2(c->prec, G, a.f2a, a.f2b, a.f1a, a.l2pl1, a.l2oct, a.f1pos, a.o_l1grid, a.o_ref1, a.o_fof1,
                         a.l1grid, tiles(c, 2), float(c->d.nqs12), dec(c, 1), enc(c, 2), c->st));
   std::swap(a.f2a, a.f2b);
@@ -374,10 +379,30 @@ static int regrid_resident(jp_ctx* c) {
 int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
-  if (a.resident && !sen0 && !sen1_by_pos) return regrid_resident(c);
+  if (a.resident && !sen0 && !sen1_by_pos) return regrid_device(c, true);
   if (int r = amr_sync_counts(c)) return r;
   drop_graph(c);
-  if (int r = snapshot_tables(c)) return r;
+  bool snapshot = true;
+  if (!sen0 && !sen1_by_pos) {
+    // One count read instead of two: the regrid runs on device-resident counts against the current pools
+    // and the counts are read once at the end. Only when a count exceeded a pool (threshold ties) is it
+    // undone - the migrations wrote the second buffers, and the snapshots still hold the tables from
+    // before - and run again below with the host reading each count and growing the pool.
+    a.dev_counts = true;
+    const int r = regrid_device(c, false);
+    a.dev_counts = false;
+    if (r) return r;
+    if (int r2 = read_counts(c)) return r2;
+    if (!a.hcnt[CNT_OVERFLOW]) return take_counts(c);
+    std::swap(a.f1a, a.f1b);
+    std::swap(a.f2a, a.f2b);
+    if (int r2 = fill(c, a.dcnt + CNT_OVERFLOW, 0, 1)) return r2;
+    a.hcnt[CNT_OVERFLOW] = 0;
+    a.counts_stale = false;
+    snapshot = false;
+  }
+  if (snapshot)
+    if (int r = snapshot_tables(c)) return r;
   HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
   if (sen0)
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;

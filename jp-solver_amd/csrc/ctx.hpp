@@ -28,6 +28,7 @@ struct AmrState {
   bool resident_req = false;  // regrids on device-resident counts requested (graphs on, sensor-driven grid)
   bool resident = false;      // ... and in effect: pools sized for them, every launch sized on the device
   bool counts_stale = false;  // a device-resident regrid ran since the host last read the counts
+  bool dev_counts = false;    // launches read their tile counts on the device (during a regrid's pipeline)
   int w1 = 8, w2 = 8;         // launch widths of the level-1 / level-2 launches while resident
   hipStream_t st2 = nullptr;  // side stream of the coarse step (first level-2 pair)
   hipEvent_t ev_fork[2] = {}, ev_join[2] = {};

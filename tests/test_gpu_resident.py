@@ -171,3 +171,40 @@ def test_amr_driver_graph_loop_matches_eager_loop():
         lines = [x for x in r.stdout.splitlines() if x.startswith("  t*=") or "enstrophy peak" in x]
         out.append(lines)
     assert len(out[0]) == 10 and out[0] == out[1]
+
+
+def rest_state(ctx, set_raw):
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float32)
+    for lvl in range(3):
+        nt = ctx.raw(lvl).size // (19 * 512)
+        set_raw(lvl, np.broadcast_to(w[None, :, None], (nt, 19, 512)).ravel())
+
+
+def test_eager_regrid_is_redone_when_ties_exceed_a_pool():
+    # the eager regrid runs on device counts with one count read at the end; a state at rest ties every
+    # tile at both thresholds, the counts exceed the pools, and the regrid is undone and run again with the
+    # host growing the pools: the oracle (same growth rule) must agree bit for bit
+    n0 = 24
+    cnb0 = (n0 // 8) ** 3
+    s0, s1 = sensors(n0, 5)
+    g = dynamic(n0, frac1=0.1, frac2=0.1)
+    o = orc.Amr(n=n0, re=1600.0, l1=orc.MODE_SENSOR, l2=orc.MODE_SENSOR, frac1=0.1, frac2=0.1,
+                budget_bytes=16.0 * 2 ** 30)
+    g.init(s0, s1)
+    assert o.init(s0, s1) == 0
+    cap1 = g.counts()["cap1"]
+    rest_state(g, lambda lvl, a: g.set_raw(a, lvl))
+    rest_state(o, lambda lvl, a: o.set_raw(lvl, a))
+    g.regrid()
+    assert o.regrid() == 0
+    c = g.counts()
+    assert c["nf1"] == 8 * cnb0 > cap1 and c["cap1"] >= c["nf1"]
+    oc = o.counts()
+    for k in ["nref0", "nf1", "nref1", "nf2", "cap1", "cap2", "hw_nf1", "hw_nf2"]:
+        assert c[k] == oc[k], (k, c[k], oc[k])
+    for t in TABLES:
+        assert np.array_equal(g.table(t), o.table(t)), t
+    g.step(2)
+    o.step(2)
+    for lvl in range(3):
+        assert np.array_equal(g.raw(lvl), o.raw(lvl)), f"level {lvl}"
