@@ -235,7 +235,24 @@ int jp_init(jp_ctx* c) {
   return JP_OK;
 }
 
+// n uniform steps: two fused steps per launch where a column shape applies, a single step for an odd rest.
+static int uniform_steps(jp_ctx* c, long n) {
+  int shape[4];
+  long s = 0;
+  if (c->step_pairs && step_pair_shape(c->prec, c->d.n, shape)) {
+    const float enc = float(c->d.scale[0]), dec = float(1.0 / c->d.scale[0]);
+    for (; s + 2 <= n; s += 2) {
+      HCK(c, dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->st));
+      std::swap(c->c0a, c->c0b);
+    }
+  }
+  for (; s < n; ++s)
+    if (int r = uniform_step_once(c)) return r;
+  return JP_OK;
+}
+
 static int eager_steps(jp_ctx* c, long n) {
+  if (c->kind == 0) return uniform_steps(c, n);
   for (long s = 0; s < n; ++s) {
     int r = c->kind == 2 ? ens_step_once(c) : c->kind == 1 ? amr_base_step(c, c->st) : uniform_step_once(c);
     if (r) return r;
@@ -257,13 +274,16 @@ static void graph_key(const jp_ctx* c, const void* key[3]) {
 
 // The captured pair of steps for the current roles, captured on first use. A few pairs are kept: the
 // two parities of the dense buffers and, with device-resident regrids (which swap both fine pairs), of
-// the fine buffers.
+// the fine buffers. Capturing leaves the dense roles as the captured steps leave them; returning a kept
+// pair sets them the same way, before its replay is enqueued.
 static int graph_for(jp_ctx* c, hipGraphExec_t* out) {
   const void* key[3];
   graph_key(c, key);
   for (const auto& g : c->graphs)
     if (g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2]) {
       *out = g.exec;
+      c->c0a = g.c0[0];
+      c->c0b = g.c0[1];
       return JP_OK;
     }
   if (c->graphs.size() >= 8) drop_graph(c);
@@ -277,7 +297,7 @@ static int graph_for(jp_ctx* c, hipGraphExec_t* out) {
   ec = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
-  c->graphs.push_back({{key[0], key[1], key[2]}, exec});
+  c->graphs.push_back({{key[0], key[1], key[2]}, exec, {c->c0a, c->c0b}});
   HCK(c, hipGraphUpload(exec, c->st));
   *out = exec;
   return JP_OK;
@@ -302,6 +322,20 @@ int jp_set_variant(jp_ctx* c, int variant) {
   if (variant < 0 || variant > 63) return fail(c, JP_ERR_ARG, "variant must lie in [0, 63]");
   c->variant = variant;
   drop_graph(c);
+  return JP_OK;
+}
+
+int jp_set_step_pairs(jp_ctx* c, int on) {
+  if (!c) return JP_ERR_ARG;
+  c->step_pairs = on != 0;
+  drop_graph(c);
+  return JP_OK;
+}
+
+int jp_step_pair_shape(const jp_ctx* c, int shape[4]) {
+  if (!c || !shape) return JP_ERR_ARG;
+  for (int i = 0; i < 4; ++i) shape[i] = 0;
+  if (c->kind == 0 && c->step_pairs) step_pair_shape(c->prec, c->d.n, shape);
   return JP_OK;
 }
 

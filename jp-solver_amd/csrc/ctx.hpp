@@ -52,6 +52,7 @@ struct jp_ctx {
   int kind = 0;  // 0 uniform, 1 AMR, 2 batched ensemble
   int device = 0, prec = 0, variant = 0;
   bool initialised = false, use_graph = false;
+  bool step_pairs = true;  // uniform lattice: two fused steps per launch where a column shape applies
   hipStream_t st = nullptr;
   std::string err;
   jp_derived d{};
@@ -66,10 +67,13 @@ struct jp_ctx {
   size_t host_out_bytes = 0;
   double* partial = nullptr;
   // Captured pairs of steps, keyed by the buffers they start from (the ping-pong roles), so that an odd
-  // step or a device-resident regrid selects another pair instead of recapturing.
+  // step or a device-resident regrid selects another pair instead of recapturing. c0 holds the dense
+  // pair's roles after the captured steps (two single steps restore them, a fused pair swaps them once),
+  // which a replay applies.
   struct Graph {
     const void* key[3];
     hipGraphExec_t exec;
+    void* c0[2];
   };
   std::vector<Graph> graphs;
   std::vector<std::pair<void*, size_t>> allocs;

@@ -200,6 +200,191 @@ __global__ __launch_bounds__(BS) void k_dense_step_vec(const typename S::raw* __
   dense_step_vec<S, V, kNtLoad, kNtStore>(fa, fb, g, omega, enc, dec, blockIdx.x * BS + threadIdx.x);
 }
 
+// ---- two fused steps per pass ----
+// A workgroup owns a TX x TY column of the lattice over TZ consecutive z-planes and streams up the column:
+// at iteration it it computes the intermediate step (t + 1) on the plane zo0 - 1 + it of an extended
+// (TX + 2) x (TY + 2) tile (phase A, each cell pulling from the lattice at t, exactly as the single
+// step), keeps it in LDS, then computes the step t + 2 of the column's plane zo0 + it - 2 from the three
+// intermediate planes around it (phase B) and writes it. The intermediate state never leaves the chip:
+// per two steps HBM sees the t-state of the extended tile read once (its x-y halo and the two extra
+// z-planes re-read by the neighbouring columns) and the t + 2 state written once, instead of two full
+// reads and writes. Every cell's arithmetic is the single step's (src/tgv3d_uniform.cu:41-61), the
+// intermediate populations pass through the storage form as a stored population would, so the result
+// is bit-identical to two single steps.
+//
+// LDS holds each direction's intermediate planes as a ring: the second step at plane o pulls direction q
+// from plane o - c_z, and the newest intermediate plane is o + 1, so a direction with c_z = -1 needs 1
+// plane, c_z = 0 needs 2 and c_z = +1 needs 3: 38 planes of the extended tile in all.
+__host__ __device__ constexpr int ring_depth(int q) { return 2 + CZ(q); }
+__host__ __device__ constexpr int ring_base(int q) {
+  int s = 0;
+  for (int i = 0; i < q; ++i) s += ring_depth(i);
+  return s;
+}
+constexpr int kRingPlanes = ring_base(Q);
+
+// Workgroup barrier that orders LDS only: the population loads of the next plane stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Column of workgroup w in a launch of n: every XCD (w % 8) gets a contiguous run of columns, so the
+// halo rows shared by neighbouring columns are read through one L2.
+__device__ __forceinline__ uint32_t xcd_column(uint32_t w, uint32_t n) {
+  const uint32_t x = w % 8, i = w / 8, q = n / 8, r = n % 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+template <class S, int TX, int TY, int BS>
+__global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __restrict__ fa,
+                                                    typename S::raw* __restrict__ fb, Dense g, uint32_t TZ,
+                                                    typename S::real omega, float enc, float dec) {
+  using raw = typename S::raw;
+  using R = typename S::real;
+  constexpr int EX = TX + 2, EY = TY + 2, E = EX * EY;
+  constexpr int PA = (E + BS - 1) / BS, PB = (TX * TY + BS - 1) / BS;
+  __shared__ raw ring[kRingPlanes * E];
+  const uint32_t N = g.N, NN = N * N;
+  const uint32_t ntx = N / TX, nty = N / TY, nt = ntx * nty * (N / TZ);
+  const uint32_t t = xcd_column(blockIdx.x, nt);
+  const uint32_t bx = t % ntx, by = (t / ntx) % nty, zo0 = (t / (ntx * nty)) * TZ;
+  const int tid = threadIdx.x;
+  // Phase-A cells of this lane: extended-tile cell e = a * BS + tid. Spare lanes of the last pass take
+  // the last cell's addresses (they load and discard), so every lane issues the same loads and the
+  // compiler can count them: the next plane's loads then stay in flight while the current one computes.
+  uint32_t colx[PA][3], coly[PA][3];
+  bool va[PA];
+#pragma unroll
+  for (int a = 0; a < PA; ++a) {
+    const int e0 = a * BS + tid;
+    va[a] = e0 < E;
+    const int e = e0 < E ? e0 : E - 1;
+    const int gx = int(bx * TX) + e % EX - 1, gy = int(by * TY) + e / EX - 1;
+    const uint32_t x = gx < 0 ? N - 1 : (gx >= int(N) ? 0 : gx), y = gy < 0 ? N - 1 : (gy >= int(N) ? 0 : gy);
+    // source column of c = +1, 0, -1 (index 1 - c): destination - c, periodic
+    colx[a][0] = x == 0 ? N - 1 : x - 1;
+    colx[a][1] = x;
+    colx[a][2] = x == N - 1 ? 0 : x + 1;
+    coly[a][0] = (y == 0 ? N - 1 : y - 1) * N;
+    coly[a][1] = y * N;
+    coly[a][2] = (y == N - 1 ? 0 : y + 1) * N;
+  }
+  auto load_plane = [&](raw(&r)[PA][Q], int it) {
+    const int gp = int(zo0) - 1 + it;
+    const uint32_t z = gp < 0 ? N - 1 : (gp >= int(N) ? 0 : gp);
+    const uint32_t zs[3] = {(z == 0 ? N - 1 : z - 1) * NN, z * NN, (z == N - 1 ? 0 : z + 1) * NN};
+#pragma unroll
+    for (int a = 0; a < PA; ++a)
+      sfor<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        r[a][q] = fa[uint64_t(q) * g.plane + zs[1 - CZ(q)] + coly[a][1 - CY(q)] + colx[a][1 - CX(q)]];
+      });
+  };
+  auto body = [&](const raw(&cur)[PA][Q], int it) {
+    const int m2 = it & 1, m3 = it % 3;
+    // phase A: intermediate plane it of the extended tile
+#pragma unroll
+    for (int a = 0; a < PA; ++a) {
+      if (va[a]) {
+        R fin[Q];
+        sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = S::template in<decltype(qc)::value>(cur[a][decltype(qc)::value], dec); });
+        const Collider<S> col(fin);
+        const int e = a * BS + tid;
+        sfor<Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int d = ring_depth(q);
+          const int slot = d == 1 ? 0 : (d == 2 ? m2 : m3);
+          ring[(ring_base(q) + slot) * E + e] = S::template out<q>(col.template post<q>(fin[q], omega), enc);
+        });
+      }
+    }
+    lds_barrier();
+    // phase B: output plane o = zo0 + it - 2 from intermediate planes it - 2, it - 1, it
+    if (it >= 2) {
+      const uint32_t o = zo0 + it - 2;
+#pragma unroll
+      for (int b = 0; b < PB; ++b) {
+        const int c = b * BS + tid;
+        if (c < TX * TY) {
+          const int ox = c % TX, oy = c / TX;
+          R fin[Q];
+          sfor<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int d = ring_depth(q);
+            // plane o - c_z: c_z = -1 the newest (slot of it), 0 that of it - 1, +1 that of it - 2
+            const int slot = d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1));
+            fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * E + (oy + 1 - CY(q)) * EX + (ox + 1 - CX(q))], dec);
+          });
+          const Collider<S> col(fin);
+          const uint64_t cell = (uint64_t(o) * N + by * TY + oy) * N + bx * TX + ox;
+          sfor<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            fb[uint64_t(q) * g.plane + cell] = S::template out<q>(col.template post<q>(fin[q], omega), enc);
+          });
+        }
+      }
+    }
+    lds_barrier();
+  };
+  // intermediate planes 0 .. TZ + 1 in pairs (TZ is even); the last prefetch repeats plane TZ + 1
+  raw ra[PA][Q], rb[PA][Q];
+  load_plane(ra, 0);
+  for (int it = 0; it < int(TZ) + 2; it += 2) {
+    load_plane(rb, it + 1);
+    body(ra, it);
+    load_plane(ra, it + 2 < int(TZ) + 2 ? it + 2 : int(TZ) + 1);
+    body(rb, it + 1);
+  }
+}
+
+// Column shapes of the fused pair per storage form, in order of preference (measured on MI355X at
+// 512^3: FP32 128 x 4 with 832 lanes 57.9 GLUPS, 64 x 8 / 704 lanes 50.5, 32 x 16 / 640 lanes 48.2,
+// against 34.1 for single steps; FP16 64 x 16 / 1024 lanes 71.5 against 64.7). BF16 (whose rounding
+// costs more instructions) and FP64 are faster in single steps.
+struct PairShape {
+  int tx, ty, bs;
+};
+bool step_pair_shape(int prec, int N, int* out4) {
+  static const PairShape f32[] = {{128, 4, 832}, {64, 8, 704}, {32, 16, 640}};
+  static const PairShape f16[] = {{64, 16, 1024}};
+  const PairShape* list = prec == 0 ? f32 : (prec == 1 ? f16 : nullptr);
+  const int n = prec == 0 ? 3 : (prec == 1 ? 1 : 0);
+  for (int i = 0; i < n; ++i) {
+    if (N % list[i].tx || N % list[i].ty) continue;
+    const int tz = N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0));
+    if (!tz) return false;
+    out4[0] = list[i].tx;
+    out4[1] = list[i].ty;
+    out4[2] = tz;
+    out4[3] = list[i].bs;
+    return true;
+  }
+  return false;
+}
+
+template <class S, int TX, int TY, int BS>
+static hipError_t pair_launch(const void* fa, void* fb, const Dense& g, int tz, double omega, float enc, float dec,
+                              hipStream_t st) {
+  using raw = typename S::raw;
+  const uint32_t nt = (g.N / TX) * (g.N / TY) * (g.N / uint32_t(tz));
+  hipLaunchKernelGGL((k_dense_step2<S, TX, TY, BS>), dim3(nt), dim3(BS), 0, st, static_cast<const raw*>(fa),
+                     static_cast<raw*>(fb), g, uint32_t(tz), typename S::real(omega), enc, dec);
+  return hipGetLastError();
+}
+hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
+                           hipStream_t st) {
+  int s[4];
+  if (!step_pair_shape(prec, int(g.N), s)) return hipErrorInvalidValue;
+  if (prec == 0) {
+    if (s[0] == 128) return pair_launch<FormF32, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
+    if (s[0] == 64) return pair_launch<FormF32, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
+    return pair_launch<FormF32, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
+  }
+  return pair_launch<FormF16, 64, 16, 1024>(fa, fb, g, s[2], omega, enc, dec, st);
+}
+
 template <class S, int BS, int V, bool NL, bool NS>
 static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, hipStream_t st) {
   using R = typename S::real;

@@ -106,7 +106,8 @@ def lib():
             "jp_regrid_with_sensor": (i, [vp, vp, vp]),
             "jp_get_counts": (i, [vp, ctypes.POINTER(Counts)]),
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
-            "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
+            "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]),
+            "jp_set_step_pairs": (i, [vp, i]), "jp_step_pair_shape": (i, [vp, vp]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
             "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -235,6 +236,16 @@ class _Ctx:
 
     def set_variant(self, v):
         self._check(lib().jp_set_variant(self.h, v))
+
+    def set_step_pairs(self, on=True):
+        """Two fused steps per launch (default where a column shape applies) or single steps."""
+        self._check(lib().jp_set_step_pairs(self.h, 1 if on else 0))
+
+    def step_pair_shape(self):
+        """(x, y, z extent, lanes) of the fused pair's column; zeros when steps run singly."""
+        a = (ctypes.c_int * 4)()
+        self._check(lib().jp_step_pair_shape(self.h, ctypes.cast(a, ctypes.c_void_p)))
+        return tuple(a)
 
     def sync(self):
         self._check(lib().jp_sync(self.h))
