@@ -2,13 +2,17 @@
 """Throughput of the D3Q19 TGV time step on one MI355X, as one JSON line.
 
 Workload: uniform periodic D3Q19 BGK Taylor-Green vortex, 512^3, FP32 populations, Re 1600, u0 0.05
-(two population buffers, 20.4 GB), the lattice's own closed-form initial state. A step is one fused
-pull stream-collide launch over the whole lattice; W untimed steps, then K steps timed with HIP events
-on the context's stream (inside the library, jp_bench). value = GLUPS (1e9 cell updates / s).
+(two population buffers, 20.4 GB), the lattice's own closed-form initial state. A step is the pull
+stream-collide over the whole lattice; the library runs two fused steps per launch (k_dense_step2: each
+workgroup streams a 128 x 4 x 128 column and keeps the intermediate step in LDS). W untimed steps, then
+K steps timed with HIP events on the context's stream (inside the library, jp_bench). value = GLUPS
+(1e9 cell updates / s).
 
-roofline: the step kernel moves 19 x 4 B read + 19 x 4 B written = 152 B per cell update; achieved =
-152 B x N^3 / (average step duration), against MI355X's 8 TB/s HBM3E peak. traffic comes from the
-committed rocprofv3 counter summary for this workload (profiles/), scaled by a copy kernel's counters.
+roofline: the algorithmic cost of a cell update is 19 x 4 B read + 19 x 4 B written = 152 B; achieved =
+152 B x 2 N^3 per launch / (average launch duration), against MI355X's 8 TB/s HBM3E peak. Because the
+intermediate step never leaves the chip, the fused launch moves fewer HBM bytes than that: traffic (HBM
+bytes per launch) comes from the committed rocprofv3 counter summary for this workload (profiles/),
+scaled by a copy kernel's counters, and extra.hbm_gbps_actual = traffic / launch duration.
 cpu_baseline: the CPU oracle (the reference step restated in OpenMP C++) on the host cores, on a
 bounded sample of the same lattice update (128^3, a few tens of steps).
 """
@@ -24,7 +28,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
 METRIC = "GLUPS + % HBM roofline, uniform & AMR TGV, FP32 vs FP16-store at matched err"
+# HBM bytes per launch of the 512^3 FP32 step: the fused two-step launch, and the single-step kernel
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json")
+TRAFFIC_FILE_SINGLE = os.path.join(ROOT, "profiles", "uniform512_fp32_single_step_traffic.json")
 
 
 def cpu_baseline(target_s=15.0):
@@ -42,12 +48,14 @@ def cpu_baseline(target_s=15.0):
                       f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
 
 
-def bench_uniform(n, prec, steps, warmup=20):
+def bench_uniform(n, prec, steps, warmup=20, pairs=True):
     import jpabi
 
     g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=jpabi.PREC_NAMES[prec])
     g.init()
+    g.set_step_pairs(pairs)
     r = g.bench(steps, warmup=warmup)
+    r["pair_shape"] = list(g.step_pair_shape())
     g.close()
     r["frac_hbm"] = r["gbps"] / PEAK_HBM_GBPS
     return r
@@ -105,6 +113,7 @@ def extras(quick=False):
     """Secondary workloads of the same metric (reported beside the headline, never as `value`)."""
     out = {}
     jobs = {
+        "uniform512_fp32_single_steps": lambda: bench_uniform(512, "fp32", 300 if quick else 500, pairs=False),
         "uniform512_fp16": lambda: bench_uniform(512, "fp16", 300 if quick else 1000),
         "uniform512_bf16": lambda: bench_uniform(512, "bf16", 300 if quick else 1000),
         # regrids every 10 coarse steps inside the timed loop (the _graph line: graphs, regrids on
@@ -171,14 +180,19 @@ def main():
     g.init()
     print(f"# uniform {n}^3 {args.prec}: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr,
           flush=True)
+    shape = g.step_pair_shape()  # (x, y, z, lanes) of the fused column; zeros for single steps
+    per_launch = 2 if shape[0] else 1
     r = g.bench(args.steps, warmup=args.warmup, graph=bool(args.graph))
     E, ens, mass = g.diag()
     g.close()
     bpu = r["bytes_per_update"]
-    achieved = bpu * n ** 3 / (r["ms_per_step"] * 1e-3) / 1e9  # GB/s of the step kernel
+    ms_launch = r["ms_per_step"] * per_launch
+    # algorithmic bytes per launch / launch duration
+    achieved = bpu * n ** 3 * per_launch / (ms_launch * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and args.prec == "fp32" and n == 512:
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILE if per_launch == 2 else TRAFFIC_FILE_SINGLE
+    if os.path.exists(tfile) and args.prec == "fp32" and n == 512:
+        with open(tfile) as f:
             t = json.load(f)
         traffic = t.get("hbm_bytes_per_launch")
     line = {
@@ -196,12 +210,18 @@ def main():
         "data": "synthetic (closed-form TGV initial state, no dataset)",
         "config": {"workload": f"uniform D3Q19 BGK TGV {n}^3 {args.prec.upper()} populations, Re 1600, u0 0.05, "
                                f"{args.steps} timed steps", "n": n, "re": 1600.0, "u0": 0.05,
-                   "precision": args.prec, "graph": bool(args.graph)},
+                   "precision": args.prec, "graph": bool(args.graph),
+                   "launch": (f"two fused steps per launch, columns {shape[0]} x {shape[1]} x {shape[2]}, "
+                              f"{shape[3]} lanes" if shape[0] else "one step per launch")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
         "extra": {"copy_kernel_gbps": copy_gbps, "attainable_hbm_gbps": attainable,
-                  "frac_of_attainable_copy": achieved / attainable["copy"], "bytes_per_update": bpu, "pool_gb": r["pool_gb"],
-                  "diag_after": [E, ens, mass]},
+                  "steps_per_launch": per_launch, "ms_per_launch": ms_launch,
+                  "hbm_gbps_actual": traffic / (ms_launch * 1e-3) / 1e9 if traffic else None,
+                  "frac_hbm_actual": traffic / (ms_launch * 1e-3) / 1e9 / PEAK_HBM_GBPS if traffic else None,
+                  "frac_of_attainable_copy_actual": (traffic / (ms_launch * 1e-3) / 1e9 / attainable["copy"]
+                                                     if traffic else None),
+                  "bytes_per_update": bpu, "pool_gb": r["pool_gb"], "diag_after": [E, ens, mass]},
     }
     if not args.no_extras:
         line["extra"]["workloads"] = extras(quick=args.quick_extras)

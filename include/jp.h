@@ -204,9 +204,9 @@ int jp_set_variant(jp_ctx* c, int variant);
 /* Two fused time steps per launch for the uniform lattice (not in the reference, which launches one
  * k_step per step, src/tgv3d_uniform.cu:166-167; a tuning knob): a workgroup streams a column of the
  * lattice and keeps the intermediate step in LDS, so HBM sees one read and one write per two steps plus
- * the column halos. On by default for FP32 and FP16 storage where a column shape divides N (a multiple
- * of 32); BF16, FP64 and other edges run single steps. Results are bit-identical either way; an odd
- * step count ends with a single step. */
+ * the column halos. On by default for FP32 storage where a column shape divides N (a multiple of 32);
+ * the 16-bit forms (faster in single steps), FP64 and other edges run single steps. Results are
+ * bit-identical either way; an odd step count ends with a single step. */
 int jp_set_step_pairs(jp_ctx* c, int on);
 /* The fused pair's column shape {x, y, z extent, lanes per workgroup}; all zero when steps run singly. */
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]);

@@ -35,7 +35,7 @@ hipError_t dense_init(int prec, void* f, const Dense& g, double u0, double kf, c
 hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
                       int variant, hipStream_t st);
 // Column shape {TX, TY, TZ, lanes} of the two-step fused pass for this form and edge; false where the
-// lattice runs single steps (BF16, FP64, or an edge no shape divides).
+// lattice runs single steps (the 16-bit forms, FP64, or an edge no shape divides).
 bool step_pair_shape(int prec, int N, int* out4);
 // Two steps fa -> (t + 1) -> fb in one launch, bit-identical to two dense_step calls; needs a shape.
 hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,

@@ -243,6 +243,28 @@ template <class S> struct Collider {
     if constexpr (S::kShift) return x - om * (x - deq<q>(e, drho, rho));
     else return x - om * (x - feq<q>(e));
   }
+  // post<q> of all 19 directions in place. The opposite directions q and q + 1 (q odd) share 3 c.u and
+  // 0.5 (3 c.u)^2: c_{q+1} = -c_q makes every term of c.u change sign exactly, and (-a) b = -(a b) in
+  // IEEE arithmetic, so each value equals post<q>'s bit for bit, with three fewer operations per pair.
+  __device__ void post_all(R* f, R om) const {
+    f[0] = post<0>(f[0], om);
+    sfor<9>([&](auto pc) {
+      constexpr int q = 1 + 2 * decltype(pc)::value;
+      constexpr int cls = q < 7 ? 1 : 2;
+      const R cu = R(3) * cdot<q>(e.ux, e.uy, e.uz);
+      const R h = R(0.5) * cu * cu;
+      R ep, em;
+      if constexpr (S::kShift) {
+        ep = W<R>(q) * (drho + rho * ((cu + h) - e.t15));
+        em = W<R>(q) * (drho + rho * ((-cu + h) - e.t15));
+      } else {
+        ep = e.wr[cls] * (((R(1) + cu) + h) - e.t15);
+        em = e.wr[cls] * (((R(1) - cu) + h) - e.t15);
+      }
+      f[q] = f[q] - om * (f[q] - ep);
+      f[q + 1] = f[q + 1] - om * (f[q + 1] - em);
+    });
+  }
 };
 
 // Runtime-q decode for kernels that gather one direction at a time.

@@ -4,8 +4,10 @@
 // Layout: 19 x-fastest planes, f[q*plane + (z*N + y)*N + x]. A wave of 64 lanes covers 64 consecutive
 // cells of one x-row, so every population load is a contiguous 256 B (FP32) run, shifted by one element
 // for directions with c_x != 0 (the partial line is shared with the neighbouring wave of the same block
-// and served by the CU's L1). Each element of every plane is read exactly once per step, so the kernel
-// moves the algorithmic 2*19 elements per cell update and nothing else.
+// and served by the CU's L1). Each element of every plane is read exactly once per step, so the single
+// step moves the algorithmic 2*19 elements per cell update and nothing else. The FP32 uniform lattice
+// runs two steps per launch instead (k_dense_step2), which keeps the intermediate step on the chip and
+// moves fewer bytes than two single steps.
 #include <algorithm>
 
 #include "dense_bodies.hpp"
@@ -290,13 +292,13 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
       if (va[a]) {
         R fin[Q];
         sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = S::template in<decltype(qc)::value>(cur[a][decltype(qc)::value], dec); });
-        const Collider<S> col(fin);
+        Collider<S>(fin).post_all(fin, omega);
         const int e = a * BS + tid;
         sfor<Q>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int d = ring_depth(q);
           const int slot = d == 1 ? 0 : (d == 2 ? m2 : m3);
-          ring[(ring_base(q) + slot) * E + e] = S::template out<q>(col.template post<q>(fin[q], omega), enc);
+          ring[(ring_base(q) + slot) * E + e] = S::template out<q>(fin[q], enc);
         });
       }
     }
@@ -317,11 +319,11 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
             const int slot = d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1));
             fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * E + (oy + 1 - CY(q)) * EX + (ox + 1 - CX(q))], dec);
           });
-          const Collider<S> col(fin);
+          Collider<S>(fin).post_all(fin, omega);
           const uint64_t cell = (uint64_t(o) * N + by * TY + oy) * N + bx * TX + ox;
           sfor<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fb[uint64_t(q) * g.plane + cell] = S::template out<q>(col.template post<q>(fin[q], omega), enc);
+            fb[uint64_t(q) * g.plane + cell] = S::template out<q>(fin[q], enc);
           });
         }
       }
@@ -339,26 +341,25 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   }
 }
 
-// Column shapes of the fused pair per storage form, in order of preference (measured on MI355X at
-// 512^3: FP32 128 x 4 with 832 lanes 57.9 GLUPS, 64 x 8 / 704 lanes 50.5, 32 x 16 / 640 lanes 48.2,
-// against 34.1 for single steps; FP16 64 x 16 / 1024 lanes 71.5 against 64.7). BF16 (whose rounding
-// costs more instructions) and FP64 are faster in single steps.
+// Column shapes of the fused pair, in order of preference (measured on MI355X at 512^3 FP32: 128 x 4
+// with 832 lanes 57.9 GLUPS, 64 x 8 / 704 lanes 50.5, 32 x 16 / 640 lanes 48.2, against 34.2 for single
+// steps). Only FP32 fuses: the 16-bit forms move half the bytes, so their fused pass (2.5 collisions per
+// cell where the single step has 1) is bound by arithmetic and ran slower than single steps in the same
+// process (FP16 64 x 16 / 1024 lanes 68.6 against 70.9 GLUPS); FP64 needs twice the LDS per cell.
 struct PairShape {
   int tx, ty, bs;
 };
 bool step_pair_shape(int prec, int N, int* out4) {
   static const PairShape f32[] = {{128, 4, 832}, {64, 8, 704}, {32, 16, 640}};
-  static const PairShape f16[] = {{64, 16, 1024}};
-  const PairShape* list = prec == 0 ? f32 : (prec == 1 ? f16 : nullptr);
-  const int n = prec == 0 ? 3 : (prec == 1 ? 1 : 0);
-  for (int i = 0; i < n; ++i) {
-    if (N % list[i].tx || N % list[i].ty) continue;
+  if (prec != 0) return false;
+  for (const PairShape& p : f32) {
+    if (N % p.tx || N % p.ty) continue;
     const int tz = N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0));
     if (!tz) return false;
-    out4[0] = list[i].tx;
-    out4[1] = list[i].ty;
+    out4[0] = p.tx;
+    out4[1] = p.ty;
     out4[2] = tz;
-    out4[3] = list[i].bs;
+    out4[3] = p.bs;
     return true;
   }
   return false;
@@ -377,12 +378,9 @@ hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, d
                            hipStream_t st) {
   int s[4];
   if (!step_pair_shape(prec, int(g.N), s)) return hipErrorInvalidValue;
-  if (prec == 0) {
-    if (s[0] == 128) return pair_launch<FormF32, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
-    if (s[0] == 64) return pair_launch<FormF32, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
-    return pair_launch<FormF32, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
-  }
-  return pair_launch<FormF16, 64, 16, 1024>(fa, fb, g, s[2], omega, enc, dec, st);
+  if (s[0] == 128) return pair_launch<FormF32, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
+  if (s[0] == 64) return pair_launch<FormF32, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
+  return pair_launch<FormF32, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
 }
 
 template <class S, int BS, int V, bool NL, bool NS>
