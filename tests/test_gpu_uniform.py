@@ -100,6 +100,46 @@ def test_fused_pairs_equal_single_steps(n):
     assert np.array_equal(a.raw(), b.raw())
 
 
+@pytest.mark.parametrize("prec", [jpabi.PREC_FP32, jpabi.PREC_FP16])
+def test_every_single_step_variant_is_bit_exact(prec):
+    """The 64 launch shapes of the single step (workgroup size, cache policy, cells per lane) compute the
+    same populations as the oracle."""
+    n = 32
+    o = orc.Uniform(n, re=1600, prec=prec)
+    o.init()
+    o.step(2)
+    ref = o.raw()
+    o2 = orc.Uniform(n, re=1600, prec=prec)
+    o2.init()
+    start = o2.raw()
+    g = jpabi.Uniform(n, re=1600, prec=prec)
+    g.set_step_pairs(False)
+    for v in range(64):
+        g.set_variant(v)
+        g.set_raw(start)
+        g.step(2)
+        assert np.array_equal(g.raw(), ref), v
+
+
+@pytest.mark.parametrize("pad", [64, 4096])
+def test_padded_planes_are_bit_exact(pad):
+    """Padding between population planes (a layout parameter) changes no result, on the fused pairs and on
+    single steps."""
+    n = 64
+    o = orc.Uniform(n, re=1600)
+    o.init()
+    g = jpabi.Uniform(n, re=1600, plane_pad=pad)
+    assert g.step_pair_shape()[0] == 64
+    g.set_raw(o.raw())
+    o.step(5)
+    g.step(5)  # two fused pairs and a single step
+    assert np.array_equal(g.raw(), o.raw())
+    g.set_step_pairs(False)
+    o.step(2)
+    g.step(2)
+    assert np.array_equal(g.raw(), o.raw())
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_diag_matches_oracle(prec):
     n = 32
