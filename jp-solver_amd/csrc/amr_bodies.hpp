@@ -31,15 +31,6 @@ __device__ __forceinline__ uint32_t dense_of_tile(const AmrGeo& G, int cb, int c
 }
 __device__ __forceinline__ size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
 
-// Tile of workgroup w in a launch of n one-tile workgroups. The dispatcher deals workgroups to the 8
-// XCDs round-robin (w -> XCD w % 8), and each XCD has its own L2: giving every XCD a contiguous run of
-// tiles keeps neighbouring tiles (the octants of one parent, whose faces and parent boxes overlap) in
-// one L2 instead of spreading them over all eight. A bijection of [0, n).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int w, int n) {
-  const int x = w % kXcds, i = w / kXcds, q = n / kXcds, r = n % kXcds;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
 
 // The lane index as a value the compiler cannot see through, taken once per iteration of a loop over
 // tiles: address arithmetic derived from it is then computed inside the iteration instead of being

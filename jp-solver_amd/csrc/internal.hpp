@@ -24,6 +24,16 @@ inline Dense make_dense(int N, uint64_t pad) {
   d.divN = make_fastdiv(uint32_t(N));
   return d;
 }
+// Work item of workgroup w in a launch of n one-item workgroups. The dispatcher deals workgroups to the 8
+// XCDs round-robin (w -> XCD w % 8), and each XCD has its own L2: giving every XCD a contiguous run of
+// items keeps neighbouring items (the octants of one parent tile, the columns of one lattice band, whose
+// faces and halos overlap) in one L2 instead of spreading them over all eight. A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+  const int x = w % kXcds, i = w / kXcds, q = n / kXcds, r = n % kXcds;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <class R> struct alignas(sizeof(R) * 4) V4 {
   R a, b, c, d;
 };

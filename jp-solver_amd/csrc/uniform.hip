@@ -232,13 +232,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Column of workgroup w in a launch of n: every XCD (w % 8) gets a contiguous run of columns, so the
-// halo rows shared by neighbouring columns are read through one L2.
-__device__ __forceinline__ uint32_t xcd_column(uint32_t w, uint32_t n) {
-  const uint32_t x = w % 8, i = w / 8, q = n / 8, r = n % 8;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 template <class S, int TX, int TY, int BS>
 __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __restrict__ fa,
                                                     typename S::raw* __restrict__ fb, Dense g, uint32_t TZ,
@@ -250,7 +243,9 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   __shared__ raw ring[kRingPlanes * E];
   const uint32_t N = g.N, NN = N * N;
   const uint32_t ntx = N / TX, nty = N / TY, nt = ntx * nty * (N / TZ);
-  const uint32_t t = xcd_column(blockIdx.x, nt);
+  // every XCD streams a contiguous run of columns, so halo rows shared by neighbouring columns are read
+  // through one L2
+  const uint32_t t = uint32_t(xcd_tile(int(blockIdx.x), int(nt)));
   const uint32_t bx = t % ntx, by = (t / ntx) % nty, zo0 = (t / (ntx * nty)) * TZ;
   const int tid = threadIdx.x;
   // Phase-A cells of this lane: extended-tile cell e = a * BS + tid. Spare lanes of the last pass take
