@@ -125,6 +125,13 @@ def extras(quick=False):
                                                      100 if quick else 300, prec="fp16"),
         "amr_gold_64_256": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, 50 if quick else 200,
                                              graph=True),
+        # the precision configuration's sizes: uniform 768^3 and dynamic AMR coarse 128 / finest 512
+        "uniform768_fp32": lambda: bench_uniform(768, "fp32", 100 if quick else 200),
+        "uniform768_fp16": lambda: bench_uniform(768, "fp16", 100 if quick else 200),
+        "uniform768_bf16": lambda: bench_uniform(768, "bf16", 100 if quick else 200),
+        "amr_dynamic_128_512": lambda: bench_amr(128, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, 50 if quick else 100),
+        "amr_dynamic_128_512_fp16": lambda: bench_amr(128, 1600.0, "sensor", "sensor", 0.3, 0.1, 10,
+                                                      50 if quick else 100, prec="fp16"),
         "ensemble32": lambda: bench_ensemble(20 if quick else 60),
         "field_output_512": lambda: bench_field(512, 2 if quick else 3),
     }
