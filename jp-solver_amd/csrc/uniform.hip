@@ -216,7 +216,10 @@ __global__ __launch_bounds__(BS) void k_dense_step_vec(const typename S::raw* __
 //
 // LDS holds each direction's intermediate planes as a ring: the second step at plane o pulls direction q
 // from plane o - c_z, and the newest intermediate plane is o + 1, so a direction with c_z = -1 needs 1
-// plane, c_z = 0 needs 2 and c_z = +1 needs 3: 38 planes of the extended tile in all.
+// plane, c_z = 0 needs 2 and c_z = +1 needs 3: 38 planes in all. A ring plane of direction q holds only
+// the TX x TY values the second step pulls, indexed by the output cell that pulls them: the extended
+// cell (ex, ey) feeds direction q of output cell (ex - 1 + c_x, ey - 1 + c_y) and nothing else, so phase
+// A stores each value at most once and phase B reads every direction at its own cell index.
 __host__ __device__ constexpr int ring_depth(int q) { return 2 + CZ(q); }
 __host__ __device__ constexpr int ring_base(int q) {
   int s = 0;
@@ -238,9 +241,9 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
                                                     typename S::real omega, float enc, float dec) {
   using raw = typename S::raw;
   using R = typename S::real;
-  constexpr int EX = TX + 2, EY = TY + 2, E = EX * EY;
-  constexpr int PA = (E + BS - 1) / BS, PB = (TX * TY + BS - 1) / BS;
-  __shared__ raw ring[kRingPlanes * E];
+  constexpr int EX = TX + 2, EY = TY + 2, E = EX * EY, T = TX * TY;
+  constexpr int PA = (E + BS - 1) / BS, PB = (T + BS - 1) / BS;
+  __shared__ raw ring[kRingPlanes * T];
   const uint32_t N = g.N, NN = N * N;
   const uint32_t ntx = N / TX, nty = N / TY, nt = ntx * nty * (N / TZ);
   // every XCD streams a contiguous run of columns, so halo rows shared by neighbouring columns are read
@@ -252,13 +255,16 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   // the last cell's addresses (they load and discard), so every lane issues the same loads and the
   // compiler can count them: the next plane's loads then stay in flight while the current one computes.
   uint32_t colx[PA][3], coly[PA][3];
+  int exa[PA], eya[PA];
   bool va[PA];
 #pragma unroll
   for (int a = 0; a < PA; ++a) {
     const int e0 = a * BS + tid;
     va[a] = e0 < E;
     const int e = e0 < E ? e0 : E - 1;
-    const int gx = int(bx * TX) + e % EX - 1, gy = int(by * TY) + e / EX - 1;
+    exa[a] = e % EX;
+    eya[a] = e / EX;
+    const int gx = int(bx * TX) + exa[a] - 1, gy = int(by * TY) + eya[a] - 1;
     const uint32_t x = gx < 0 ? N - 1 : (gx >= int(N) ? 0 : gx), y = gy < 0 ? N - 1 : (gy >= int(N) ? 0 : gy);
     // source column of c = +1, 0, -1 (index 1 - c): destination - c, periodic
     colx[a][0] = x == 0 ? N - 1 : x - 1;
@@ -288,12 +294,13 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
         R fin[Q];
         sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = S::template in<decltype(qc)::value>(cur[a][decltype(qc)::value], dec); });
         Collider<S>(fin).post_all(fin, omega);
-        const int e = a * BS + tid;
         sfor<Q>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int d = ring_depth(q);
           const int slot = d == 1 ? 0 : (d == 2 ? m2 : m3);
-          ring[(ring_base(q) + slot) * E + e] = S::template out<q>(fin[q], enc);
+          const int ox = exa[a] - 1 + CX(q), oy = eya[a] - 1 + CY(q);  // the output cell pulling q from here
+          if (unsigned(ox) < unsigned(TX) && unsigned(oy) < unsigned(TY))
+            ring[(ring_base(q) + slot) * T + oy * TX + ox] = S::template out<q>(fin[q], enc);
         });
       }
     }
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 #pragma unroll
       for (int b = 0; b < PB; ++b) {
         const int c = b * BS + tid;
-        if (c < TX * TY) {
+        if (c < T) {
           const int ox = c % TX, oy = c / TX;
           R fin[Q];
           sfor<Q>([&](auto qc) {
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
             constexpr int d = ring_depth(q);
             // plane o - c_z: c_z = -1 the newest (slot of it), 0 that of it - 1, +1 that of it - 2
             const int slot = d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1));
-            fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * E + (oy + 1 - CY(q)) * EX + (ox + 1 - CX(q))], dec);
+            fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * T + c], dec);
           });
           Collider<S>(fin).post_all(fin, omega);
           const uint64_t cell = (uint64_t(o) * N + by * TY + oy) * N + bx * TX + ox;
@@ -337,8 +344,8 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 }
 
 // Column shapes of the fused pair, in order of preference (measured on MI355X at 512^3 FP32: 128 x 4
-// with 832 lanes 57.9 GLUPS, 64 x 8 / 704 lanes 50.5, 32 x 16 / 640 lanes 48.2, against 34.2 for single
-// steps). Only FP32 fuses: the 16-bit forms move half the bytes, so their fused pass (2.5 collisions per
+// with 832 lanes 58.3 GLUPS, 64 x 8 / 704 lanes 54.3, against 34.2 for single steps; taller columns need
+// two phase-A passes per lane, whose doubled prefetch registers spill at 1024 lanes). Only FP32 fuses: the 16-bit forms move half the bytes, so their fused pass (2.5 collisions per
 // cell where the single step has 1) is bound by arithmetic and ran slower than single steps in the same
 // process (FP16 64 x 16 / 1024 lanes 68.6 against 70.9 GLUPS); FP64 needs twice the LDS per cell.
 struct PairShape {
