@@ -170,7 +170,8 @@ int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
 /* Picks the workgroup count of every batched launch by timing candidates on the current state (each
  * timed kernel is re-run on unchanged inputs, so the state is unchanged); result in groups[JP_ENS_KERNELS]
  * (dense step, level-1 fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step,
- * level-1 step + restriction). */
+ * level-1 step + restriction). The two launches with a restriction write the parent level, so they are
+ * not timed: they always run one workgroup per tile, which overlaps the restriction's tail of each tile. */
 #define JP_ENS_KERNELS 7
 int jp_autotune(jp_ctx* ens, int* groups);
 

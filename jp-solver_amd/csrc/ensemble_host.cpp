@@ -29,6 +29,14 @@ struct EnsState {
   long steps = 0;
 };
 
+// Workgroups of a batched launch. A workgroup of the fused step + restriction kernels ends every item with
+// one wave finishing the tile's 64 parents while the others wait, so a workgroup walking several items
+// would serialise those tails; with one workgroup per item the dispatcher overlaps them with other
+// items. The other kernels take their autotuned count.
+static int ens_groups(const EnsState& e, int k) {
+  return (k == EK_STEP1R || k == EK_STEP2R) ? std::max(1, e.n[k]) : e.groups[k];
+}
+
 
 static MemberDesc describe(const jp_ctx* m) {
   MemberDesc d;
@@ -164,7 +172,7 @@ static void swap_all(EnsState& e, int level) {
 int ens_step_once(jp_ctx* c) {
   EnsState& e = *c->ens;
   auto L = [&](EnsKernel k, int pa, int pb) -> hipError_t {
-    return ens_launch(c->prec, k, e.dmd, e.items[k], e.n[k], e.groups[k], pa, pb, c->st);
+    return ens_launch(c->prec, k, e.dmd, e.items[k], e.n[k], ens_groups(e, k), pa, pb, c->st);
   };
   HCK(c, L(EK_DENSE, e.p0, 0));
   swap_all(e, 0);
@@ -232,7 +240,7 @@ int ens_autotune(jp_ctx* c, int* out) {
   HCK(c, hipEventCreate(&a));
   HCK(c, hipEventCreate(&b));
   for (int k = 0; k < EK_COUNT; ++k) {
-    if (k == EK_STEP2R || k == EK_STEP1R) continue;  // they write the parent level: tuned as their unfused twins
+    if (k == EK_STEP2R || k == EK_STEP1R) continue;  // they write the parent level: one workgroup per item
     if (e.n[k] == 0) {
       e.groups[k] = cu * 4;
       continue;
@@ -260,13 +268,11 @@ int ens_autotune(jp_ctx* c, int* out) {
     }
     e.groups[k] = bestg;
   }
-  e.groups[EK_STEP2R] = e.groups[EK_STEP2];
-  e.groups[EK_STEP1R] = e.groups[EK_STEP1];
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   drop_graph(c);
   if (out)
-    for (int k = 0; k < EK_COUNT; ++k) out[k] = e.groups[k];
+    for (int k = 0; k < EK_COUNT; ++k) out[k] = ens_groups(e, k);
   return JP_OK;
 }
 
