@@ -110,6 +110,18 @@ __device__ __forceinline__ void collide_store(typename S::real (&fin)[Q], typena
 // dk): the even wave adds its four with cross-lane reads and leaves the partial sum in LDS, the odd wave
 // adds its four and leaves the mean in LDS, and one full wave finishes the 64 parents (a lane each).
 // Call with all 512 lanes.
+// Lane i receives lane i + N of its 16-lane row (DPP row_shl:N, a VALU move with no LDS round trip);
+// lanes whose source lies past the row's end keep v.
+template <int N> __device__ __forceinline__ float row_down(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x100 + N, 0xF, 0xF, false));
+}
+template <int N> __device__ __forceinline__ double row_down(double v) {
+  const int2 h = *reinterpret_cast<const int2*>(&v);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(h.x, h.x, 0x100 + N, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_update_dpp(h.y, h.y, 0x100 + N, 0xF, 0xF, false);
+  return *reinterpret_cast<const double*>(&r);
+}
 template <class R> struct RestrictLds {
   R part[HALF][Q][16];  // partial sums per plane pair, direction, parent of the pair's 4x4 layer
   R avg[Q][64];         // complete child means per direction and parent (pk * 16 + pj * 4 + pi)
@@ -147,11 +159,12 @@ __device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q],
   const int k = cell >> 6, lane = cell & 63, i = lane & 7, j = lane >> 3;  // k is uniform in a wave
   const bool corner = !(i & 1) && !(j & 1);
   const int pp = (j >> 1) * 4 + (i >> 1);
+  // a corner lane (even i, even j) reads lanes +1, +8, +9: always in its own 16-lane row (rows hold j = 2m, 2m + 1)
   if (!(k & 1)) {  // children c = 0..3 of plane 2pk
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       const R t = R(0.125) * kept[q];
-      const R t1 = __shfl_down(t, 1, 64), t8 = __shfl_down(t, 8, 64), t9 = __shfl_down(t, 9, 64);
+      const R t1 = row_down<1>(t), t8 = row_down<8>(t), t9 = row_down<9>(t);
       if (corner) L.part[k >> 1][q][pp] = ((((R(0) + t) + t1) + t8) + t9);
     });
   }
@@ -160,7 +173,7 @@ __device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q],
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       const R t = R(0.125) * kept[q];
-      const R t1 = __shfl_down(t, 1, 64), t8 = __shfl_down(t, 8, 64), t9 = __shfl_down(t, 9, 64);
+      const R t1 = row_down<1>(t), t8 = row_down<8>(t), t9 = row_down<9>(t);
       if (corner) L.avg[q][(k >> 1) * 16 + pp] = (((L.part[k >> 1][q][pp] + t) + t1) + t8) + t9;
     });
   }
