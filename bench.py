@@ -109,6 +109,22 @@ def bench_field(n=512, reps=3):
     return {"n": n, "ms_per_frame": dt * 1e3, "bytes_to_host": out_bytes, "host_gbps": out_bytes / dt / 1e9}
 
 
+def bench_diag(n=512, reps=5):
+    """Energy, enstrophy and mass of the uniform lattice: macroscopic field and FP64 reduction on the device,
+    three doubles to the host (jp_diag), the per-sample cost of the driver's diagnostics."""
+    import jpabi
+
+    g = jpabi.Uniform(n, re=1600.0, u0=0.05)
+    g.init()
+    g.diag()  # allocates the field and partial buffers
+    t0 = time.time()
+    for _ in range(reps):
+        g.diag()
+    dt = (time.time() - t0) / reps
+    g.close()
+    return {"n": n, "ms_per_sample": dt * 1e3, "field_bytes": 16 * n ** 3}
+
+
 def extras(quick=False):
     """Secondary workloads of the same metric (reported beside the headline, never as `value`)."""
     out = {}
@@ -134,6 +150,7 @@ def extras(quick=False):
                                                       50 if quick else 100, prec="fp16"),
         "ensemble32": lambda: bench_ensemble(20 if quick else 60),
         "field_output_512": lambda: bench_field(512, 2 if quick else 3),
+        "diag_512": lambda: bench_diag(512, 3 if quick else 5),
     }
     for k, fn in jobs.items():
         t0 = time.time()
@@ -142,8 +159,8 @@ def extras(quick=False):
         except Exception as ex:  # an extra must not hide the headline measurement
             out[k] = {"error": str(ex)}
         out[k]["wall_s"] = time.time() - t0
-        print(f"# extra {k}: {out[k].get('glups', out[k].get('ms_per_frame', float('nan'))):.3f}", file=sys.stderr,
-              flush=True)
+        v = out[k].get("glups", out[k].get("ms_per_frame", out[k].get("ms_per_sample", float("nan"))))
+        print(f"# extra {k}: {v:.3f}", file=sys.stderr, flush=True)
     return out
 
 
@@ -176,7 +193,8 @@ def main():
               "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
               "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),
-              "field": lambda: bench_field(args.n, 3)}[args.only]
+              "field": lambda: bench_field(args.n, 3),
+              "diag": lambda: bench_diag(args.n, 5)}[args.only]
         print(json.dumps({"workload": args.only, **fn()}), flush=True)
         return
     prec = jpabi.PREC_NAMES[args.prec]
