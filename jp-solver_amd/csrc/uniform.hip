@@ -282,7 +282,10 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
     for (int a = 0; a < PA; ++a)
       sfor<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        r[a][q] = fa[uint64_t(q) * g.plane + zs[1 - CZ(q)] + coly[a][1 - CY(q)] + colx[a][1 - CX(q)]];
+        // workgroup-uniform base of the source plane (scalar registers) plus the lane's 32-bit byte offset
+        // within it: the load takes the scalar-base address form and needs no 64-bit lane arithmetic
+        const char* base = reinterpret_cast<const char*>(fa + (uint64_t(q) * g.plane + zs[1 - CZ(q)]));
+        r[a][q] = *reinterpret_cast<const raw*>(base + (coly[a][1 - CY(q)] + colx[a][1 - CX(q)]) * uint32_t(sizeof(raw)));
       });
   };
   auto body = [&](const raw(&cur)[PA][Q], int it) {
