@@ -81,17 +81,19 @@ def test_fused_step_pairs_are_bit_exact(prec, n):
         assert np.array_equal(g.raw(), o.raw()), k
 
 
-@pytest.mark.parametrize("n", [256, 192])
-def test_fused_pairs_equal_single_steps(n):
-    """256^3 (the 128 x 4 column shape) and 192^3 (64 x 8): the fused pass equals two single-step
-    launches, bit for bit, also through graph replays."""
+@pytest.mark.parametrize("n,shape", [(256, (128, 4, 128)), (192, (64, 8, 64)), (160, (32, 16, 32)),
+                                     (224, (32, 16, 32))])
+def test_fused_pairs_equal_single_steps(n, shape):
+    """Every column shape and z-segment length (128 x 4 x 128 at 256^3, 64 x 8 x 64 at 192^3, 32 x 16 x 32
+    at 160^3 and 224^3): the fused pass equals two single-step launches, bit for bit, also through graph
+    replays."""
     a = jpabi.Uniform(n)
     b = jpabi.Uniform(n)
     a.init()
     b.set_raw(a.raw())
     b.set_step_pairs(False)
     assert b.step_pair_shape() == (0, 0, 0, 0)
-    assert a.step_pair_shape()[:3] == ((128, 4, 128) if n == 256 else (64, 8, 64))
+    assert a.step_pair_shape()[:3] == shape
     a.step(6)
     b.step(6)
     assert np.array_equal(a.raw(), b.raw())
