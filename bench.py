@@ -125,6 +125,23 @@ def bench_diag(n=512, reps=5):
     return {"n": n, "ms_per_sample": dt * 1e3, "field_bytes": 16 * n ** 3}
 
 
+def bench_amr_diag(reps=5):
+    """Energy, enstrophy and mass of the dynamic AMR case (coarse 96, finest 384): composite field on the finest
+    grid from the three levels and the FP64 reduction on the device, three doubles to the host (jp_diag)."""
+    import jpabi
+
+    a = jpabi.Amr(96, re=1600.0, u0=0.05, l1=jpabi.MODE_NAMES["sensor"], l2=jpabi.MODE_NAMES["sensor"], frac1=0.3,
+                  frac2=0.1, adapt=10)
+    a.init()
+    a.diag()  # allocates the composite field
+    t0 = time.time()
+    for _ in range(reps):
+        a.diag()
+    dt = (time.time() - t0) / reps
+    a.close()
+    return {"n_finest": 384, "ms_per_sample": dt * 1e3, "field_bytes": 16 * 384 ** 3}
+
+
 def extras(quick=False):
     """Secondary workloads of the same metric (reported beside the headline, never as `value`)."""
     out = {}
@@ -151,6 +168,7 @@ def extras(quick=False):
         "ensemble32": lambda: bench_ensemble(20 if quick else 60),
         "field_output_512": lambda: bench_field(512, 2 if quick else 3),
         "diag_512": lambda: bench_diag(512, 3 if quick else 5),
+        "amr_diag_96_384": lambda: bench_amr_diag(3 if quick else 5),
     }
     for k, fn in jobs.items():
         t0 = time.time()
@@ -194,7 +212,8 @@ def main():
               "ensemble": lambda: bench_ensemble(args.steps),
               "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),
               "field": lambda: bench_field(args.n, 3),
-              "diag": lambda: bench_diag(args.n, 5)}[args.only]
+              "diag": lambda: bench_diag(args.n, 5),
+              "amr-diag": lambda: bench_amr_diag(5)}[args.only]
         print(json.dumps({"workload": args.only, **fn()}), flush=True)
         return
     prec = jpabi.PREC_NAMES[args.prec]
