@@ -48,8 +48,12 @@ int main(int argc, char** argv) {
   if (bench) {
     jp_bench_result r;
     if (int rc = jp_bench(c, bsteps, 20, graph ? 1 : 0, &r)) return die(c, rc);
-    std::printf("[bench%s] uniform N=%d: %.3f s  work=%.3e updates  %.3f GLUPS  %.1f GB/s (%.1f%% of 8 TB/s HBM)  pools %.3f GB\n",
-                graph ? "+graph" : "", p.n, r.ms_total / 1e3, r.work, r.glups, r.gbps, r.gbps / 80.0, r.pool_gb);
+    int shape[4];
+    jp_step_pair_shape(c, shape);
+    std::printf("[bench%s] uniform N=%d: %.3f s  work=%.3e updates  %.3f GLUPS  %.1f GB/s at %.0f B/update (%.1f%% of 8 TB/s "
+                "HBM)  pools %.3f GB  %s\n",
+                graph ? "+graph" : "", p.n, r.ms_total / 1e3, r.work, r.glups, r.gbps, r.bytes_per_update, r.gbps / 80.0,
+                r.pool_gb, shape[0] ? "(two fused steps per launch: the intermediate step stays on chip)" : "(one step per launch)");
     jp_destroy(c);
     return 0;
   }
