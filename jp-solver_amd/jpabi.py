@@ -85,11 +85,15 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):  # nothing built yet: build the gfx950 library (hipcc) once
-            r = subprocess.run(["make", "-j", str(min(8, os.cpu_count() or 1)), "-C", HERE], capture_output=True,
-                               text=True)
-            if r.returncode != 0 or not os.path.exists(LIB_PATH):
-                raise RuntimeError(f"{LIB_PATH} is missing and `make -C {HERE}` failed; there is no fallback "
-                                   f"path:\n{r.stdout[-2000:]}{r.stderr[-2000:]}")
+            try:
+                r = subprocess.run(["make", "-j", str(min(8, os.cpu_count() or 1)), "-C", HERE], capture_output=True,
+                                   text=True)
+                log = r.stdout[-2000:] + r.stderr[-2000:]
+            except OSError as e:
+                log = str(e)
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is missing and `make -C {HERE}` did not build it; there is no "
+                                   f"fallback path:\n{log}")
         L = ctypes.CDLL(LIB_PATH)
         vp, i, l, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
         sig = {
