@@ -4,7 +4,8 @@
 Workload: uniform periodic D3Q19 BGK Taylor-Green vortex, 512^3, FP32 populations, Re 1600, u0 0.05
 (two population buffers, 20.4 GB), the lattice's own closed-form initial state. A step is the pull
 stream-collide over the whole lattice; the library runs two fused steps per launch (k_dense_step2: each
-workgroup streams a 128 x 4 x 128 column and keeps the intermediate step in LDS). W untimed steps, then
+workgroup streams a 128 x 4 x 128 column and keeps the intermediate step in LDS), the launch form picked
+by the library's autotuner (jp_autotune) before the timed steps. W untimed steps, then
 K steps timed with HIP events on the context's stream (inside the library, jp_bench). value = GLUPS
 (1e9 cell updates / s).
 
@@ -222,9 +223,10 @@ def main():
     attainable = jpabi.probe_bandwidth(4 << 30, iters=10)
     g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=prec)
     g.init()
-    print(f"# uniform {n}^3 {args.prec}: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr,
-          flush=True)
-    shape = g.step_pair_shape()  # (x, y, z, lanes) of the fused column; zeros for single steps
+    # the library's autotuner times single steps and the fused column shapes that divide N, keeps the fastest
+    shape = g.autotune()  # (x, y, z, lanes) of the fused column; zeros for single steps
+    print(f"# uniform {n}^3 {args.prec}: launch form {shape}, warmup {args.warmup}, timing {args.steps} steps",
+          file=sys.stderr, flush=True)
     per_launch = 2 if shape[0] else 1
     r = g.bench(args.steps, warmup=args.warmup, graph=bool(args.graph))
     E, ens, mass = g.diag()
@@ -235,7 +237,9 @@ def main():
     achieved = bpu * n ** 3 * per_launch / (ms_launch * 1e-3) / 1e9
     traffic = None
     tfile = TRAFFIC_FILE if per_launch == 2 else TRAFFIC_FILE_SINGLE
-    if os.path.exists(tfile) and args.prec == "fp32" and n == 512:
+    # the counter summaries were taken with the 128 x 4 x 128 column and with single steps
+    profiled = shape == (0, 0, 0, 0) or tuple(shape) == (128, 4, 128, 832)
+    if os.path.exists(tfile) and args.prec == "fp32" and n == 512 and profiled:
         with open(tfile) as f:
             t = json.load(f)
         traffic = t.get("hbm_bytes_per_launch")
@@ -256,7 +260,7 @@ def main():
                                f"{args.steps} timed steps", "n": n, "re": 1600.0, "u0": 0.05,
                    "precision": args.prec, "graph": bool(args.graph),
                    "launch": (f"two fused steps per launch, columns {shape[0]} x {shape[1]} x {shape[2]}, "
-                              f"{shape[3]} lanes" if shape[0] else "one step per launch")},
+                              f"{shape[3]} lanes" if shape[0] else "one step per launch") + " (picked by jp_autotune)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
         "extra": {"copy_kernel_gbps": copy_gbps, "attainable_hbm_gbps": attainable,

@@ -167,13 +167,18 @@ typedef struct {
 int jp_ensemble_create(const jp_member* members, int count, int prec, double maxgb, int device, jp_ctx** out);
 /* Borrowed handle of member `index` (owned by the ensemble, valid until jp_destroy of the ensemble). */
 int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
-/* Picks the workgroup count of every batched launch by timing candidates on the current state (each
- * timed kernel is re-run on unchanged inputs, so the state is unchanged); result in groups[JP_ENS_KERNELS]
- * (dense step, level-1 fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step,
- * level-1 step + restriction). The two launches with a restriction write the parent level, so they are
- * not timed: they always run one workgroup per tile, which overlaps the restriction's tail of each tile. */
+/* Autotuner (not in the reference; the reference has fixed launch shapes). Candidates are timed on the
+ * current state; each timed kernel is re-run on unchanged inputs, so the state is unchanged.
+ * - Ensemble: picks the workgroup count of every batched launch; result in groups[JP_ENS_KERNELS] (dense
+ *   step, level-1 fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step, level-1 step
+ *   + restriction). The two launches with a restriction write the parent level, so they are not timed:
+ *   they always run one workgroup per tile, which overlaps each tile's restriction tail.
+ * - Uniform lattice: picks how steps run, as single steps or as fused pairs with one of the column shapes
+ *   that divide the edge, and keeps the fastest. groups[0..3] receive the chosen column shape (x, y,
+ *   z extent, lanes; zeros for single steps) and groups[4..6] zero.
+ * An AMR context has no candidates (JP_ERR_STATE). */
 #define JP_ENS_KERNELS 7
-int jp_autotune(jp_ctx* ens, int* groups);
+int jp_autotune(jp_ctx* c, int* groups);
 
 int jp_get_counts(jp_ctx* c, jp_counts* out);
 /* Copies table `which` (JP_TABLE_*) to dst (capacity cap elements); *n receives its length. */
@@ -207,7 +212,8 @@ int jp_set_variant(jp_ctx* c, int variant);
  * lattice and keeps the intermediate step in LDS, so HBM sees one read and one write per two steps plus
  * the column halos. On by default for FP32 storage where a column shape divides N (a multiple of 32);
  * the 16-bit forms (faster in single steps), FP64 and other edges run single steps. Results are
- * bit-identical either way; an odd step count ends with a single step. */
+ * bit-identical either way; an odd step count ends with a single step. Resets the column shape to the
+ * first in preference order that divides N (jp_autotune may pick another). */
 int jp_set_step_pairs(jp_ctx* c, int on);
 /* The fused pair's column shape {x, y, z extent, lanes per workgroup}; all zero when steps run singly. */
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]);

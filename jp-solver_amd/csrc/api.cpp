@@ -216,12 +216,57 @@ int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member) {
   return *member ? JP_OK : fail(ens, JP_ERR_ARG, "member index out of range");
 }
 
-int jp_autotune(jp_ctx* ens, int* groups) {
-  if (!ens) return JP_ERR_ARG;
-  if (ens->kind != 2) return fail(ens, JP_ERR_STATE, "autotuning applies to the batched ensemble");
-  if (!ens->initialised) return fail(ens, JP_ERR_STATE, "jp_autotune before jp_init");
-  HCK(ens, hipSetDevice(ens->device));
-  return ens_autotune(ens, groups);
+// Uniform lattice: times two steps in every form the lattice can take (single steps, and the fused pair
+// with each column shape that divides the edge), each launch re-run from the current state into the other
+// buffer so that the state is unchanged, and keeps the fastest (median of 3 after one untimed launch).
+static int uniform_autotune(jp_ctx* c, int* out) {
+  const float enc = float(c->d.scale[0]), dec = float(1.0 / c->d.scale[0]);
+  hipEvent_t a, b;
+  HCK(c, hipEventCreate(&a));
+  HCK(c, hipEventCreate(&b));
+  auto launch = [&](int cand) -> hipError_t {  // two steps of candidate cand (-1: single steps)
+    if (cand >= 0) return dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, cand, c->st);
+    for (int k = 0; k < 2; ++k)
+      if (hipError_t e = dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->variant, c->st)) return e;
+    return hipSuccess;
+  };
+  float best = 1e30f;
+  int pick = -1, shape[4] = {0, 0, 0, 0};
+  for (int cand = -1; cand < step_pair_shape_count(); ++cand) {
+    int s[4] = {0, 0, 0, 0};
+    if (cand >= 0 && !step_pair_shape(c->prec, c->d.n, cand, s)) continue;
+    HCK(c, launch(cand));
+    float t[3];
+    for (int r = 0; r < 3; ++r) {
+      HCK(c, hipEventRecord(a, c->st));
+      HCK(c, launch(cand));
+      HCK(c, hipEventRecord(b, c->st));
+      HCK(c, hipEventSynchronize(b));
+      HCK(c, hipEventElapsedTime(&t[r], a, b));
+    }
+    std::sort(t, t + 3);
+    if (t[1] < best) {
+      best = t[1];
+      pick = cand;
+      for (int i = 0; i < 4; ++i) shape[i] = s[i];
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->step_pairs = pick >= 0;
+  c->pair_pick = pick;
+  drop_graph(c);
+  if (out)
+    for (int i = 0; i < JP_ENS_KERNELS; ++i) out[i] = i < 4 ? shape[i] : 0;
+  return JP_OK;
+}
+
+int jp_autotune(jp_ctx* c, int* groups) {
+  if (!c) return JP_ERR_ARG;
+  if (c->kind == 1) return fail(c, JP_ERR_STATE, "autotuning applies to the batched ensemble and the uniform lattice");
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_autotune before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  return c->kind == 2 ? ens_autotune(c, groups) : uniform_autotune(c, groups);
 }
 
 int jp_init(jp_ctx* c) {
@@ -239,10 +284,10 @@ int jp_init(jp_ctx* c) {
 static int uniform_steps(jp_ctx* c, long n) {
   int shape[4];
   long s = 0;
-  if (c->step_pairs && step_pair_shape(c->prec, c->d.n, shape)) {
+  if (c->step_pairs && step_pair_shape(c->prec, c->d.n, c->pair_pick, shape)) {
     const float enc = float(c->d.scale[0]), dec = float(1.0 / c->d.scale[0]);
     for (; s + 2 <= n; s += 2) {
-      HCK(c, dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->st));
+      HCK(c, dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->pair_pick, c->st));
       std::swap(c->c0a, c->c0b);
     }
   }
@@ -328,6 +373,7 @@ int jp_set_variant(jp_ctx* c, int variant) {
 int jp_set_step_pairs(jp_ctx* c, int on) {
   if (!c) return JP_ERR_ARG;
   c->step_pairs = on != 0;
+  c->pair_pick = -1;
   drop_graph(c);
   return JP_OK;
 }
@@ -335,7 +381,7 @@ int jp_set_step_pairs(jp_ctx* c, int on) {
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]) {
   if (!c || !shape) return JP_ERR_ARG;
   for (int i = 0; i < 4; ++i) shape[i] = 0;
-  if (c->kind == 0 && c->step_pairs) step_pair_shape(c->prec, c->d.n, shape);
+  if (c->kind == 0 && c->step_pairs) step_pair_shape(c->prec, c->d.n, c->pair_pick, shape);
   return JP_OK;
 }
 

@@ -53,6 +53,7 @@ struct jp_ctx {
   int device = 0, prec = 0, variant = 0;
   bool initialised = false, use_graph = false;
   bool step_pairs = true;  // uniform lattice: two fused steps per launch where a column shape applies
+  int pair_pick = -1;      // column shape of the fused pairs (-1: the first in preference order that fits)
   hipStream_t st = nullptr;
   std::string err;
   jp_derived d{};

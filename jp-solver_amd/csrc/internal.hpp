@@ -44,12 +44,14 @@ hipError_t dense_init(int prec, void* f, const Dense& g, double u0, double kf, c
                       hipStream_t st);
 hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
                       int variant, hipStream_t st);
-// Column shape {TX, TY, TZ, lanes} of the two-step fused pass for this form and edge; false where the
-// lattice runs single steps (the 16-bit forms, FP64, or an edge no shape divides).
-bool step_pair_shape(int prec, int N, int* out4);
+// Column shape {TX, TY, TZ, lanes} of the two-step fused pass for this form and edge: shape `pick` of the
+// list, or with pick < 0 the first that divides the edge; false where none applies (the 16-bit forms,
+// FP64, an edge no shape divides, or a pick that does not divide it): the lattice runs single steps.
+int step_pair_shape_count();
+bool step_pair_shape(int prec, int N, int pick, int* out4);
 // Two steps fa -> (t + 1) -> fb in one launch, bit-identical to two dense_step calls; needs a shape.
 hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
-                           hipStream_t st);
+                           int pick, hipStream_t st);
 // macro field (rho, u, v, w): float4 for FP32/FP16/BF16 storage, double4 for FP64 storage
 hipError_t dense_macro(int prec, const void* f, const Dense& g, float dec, void* field, hipStream_t st);
 // Q-criterion, speed and (when vel != nullptr) velocity of an N^3 field of (rho, u, v, w) (field.hip)

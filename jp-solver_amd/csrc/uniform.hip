@@ -354,13 +354,15 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 struct PairShape {
   int tx, ty, bs;
 };
-bool step_pair_shape(int prec, int N, int* out4) {
-  static const PairShape f32[] = {{128, 4, 832}, {64, 8, 704}, {32, 16, 640}};
+static const PairShape kPairShapes[] = {{128, 4, 832}, {64, 8, 704}, {32, 16, 640}};
+int step_pair_shape_count() { return int(sizeof(kPairShapes) / sizeof(kPairShapes[0])); }
+bool step_pair_shape(int prec, int N, int pick, int* out4) {
   if (prec != 0) return false;
-  for (const PairShape& p : f32) {
-    if (N % p.tx || N % p.ty) continue;
-    const int tz = N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0));
-    if (!tz) return false;
+  const int tz = N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0));
+  if (!tz) return false;
+  for (int i = 0; i < step_pair_shape_count(); ++i) {
+    const PairShape& p = kPairShapes[i];
+    if ((pick >= 0 && i != pick) || N % p.tx || N % p.ty) continue;
     out4[0] = p.tx;
     out4[1] = p.ty;
     out4[2] = tz;
@@ -380,9 +382,9 @@ static hipError_t pair_launch(const void* fa, void* fb, const Dense& g, int tz, 
   return hipGetLastError();
 }
 hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
-                           hipStream_t st) {
+                           int pick, hipStream_t st) {
   int s[4];
-  if (!step_pair_shape(prec, int(g.N), s)) return hipErrorInvalidValue;
+  if (!step_pair_shape(prec, int(g.N), pick, s)) return hipErrorInvalidValue;
   if (s[0] == 128) return pair_launch<FormF32, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
   if (s[0] == 64) return pair_launch<FormF32, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
   return pair_launch<FormF32, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);

@@ -273,6 +273,14 @@ class Uniform(_Ctx):
         self.h = h.value
         self._check(rc)
 
+    def autotune(self):
+        """Times single steps and every fused column shape that divides N on the current state (which it
+        leaves unchanged) and keeps the fastest; returns its shape (x, y, z extent, lanes), zeros for single
+        steps."""
+        g = (ctypes.c_int * ENS_KERNELS)()
+        self._check(lib().jp_autotune(self.h, g))
+        return tuple(g[:4])
+
 
 def _fptr(a):
     if a is None:

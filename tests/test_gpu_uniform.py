@@ -102,6 +102,38 @@ def test_fused_pairs_equal_single_steps(n, shape):
     assert np.array_equal(a.raw(), b.raw())
 
 
+@pytest.mark.parametrize("n,prec", [(256, jpabi.PREC_FP32), (96, jpabi.PREC_FP32), (64, jpabi.PREC_FP16)])
+def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
+    """The uniform autotuner times single steps and the fused column shapes that divide N, leaves the state
+    unchanged, keeps one of them, and the lattice then steps bit-exactly as before."""
+    o = orc.Uniform(n, re=1600, prec=prec)
+    o.init()
+    g = jpabi.Uniform(n, re=1600, prec=prec)
+    g.set_raw(o.raw())
+    shape = g.autotune()
+    assert np.array_equal(g.raw(), o.raw())
+    assert g.step_pair_shape() == shape
+    valid = {(0, 0, 0, 0)}
+    if prec == jpabi.PREC_FP32:
+        valid |= {s for s in [(128, 4, 128, 832), (64, 8, 64, 704), (32, 16, 32, 640), (128, 4, 64, 832),
+                              (64, 8, 128, 704), (32, 16, 128, 640), (32, 16, 64, 640)]
+                  if n % s[0] == 0 and n % s[1] == 0 and n % s[2] == 0}
+    assert shape in valid
+    o.step(5)
+    g.step(5)
+    assert np.array_equal(g.raw(), o.raw())
+
+
+def test_autotune_rejects_amr():
+    """An AMR context has nothing to autotune: jp_autotune reports a state error (called through the
+    uniform binding's method, which only forwards the handle)."""
+    a = jpabi.Amr(16, l1=jpabi.MODE_ALL, l2=jpabi.MODE_ALL)
+    a.init()
+    with pytest.raises(jpabi.JPError) as e:
+        jpabi.Uniform.autotune(a)
+    assert e.value.code == jpabi.JP_ERR_STATE
+
+
 @pytest.mark.parametrize("prec", [jpabi.PREC_FP32, jpabi.PREC_FP16])
 def test_every_single_step_variant_is_bit_exact(prec):
     """The 64 launch shapes of the single step (workgroup size, cache policy, cells per lane) compute the
