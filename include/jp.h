@@ -165,13 +165,18 @@ typedef struct {
  * jp_regrid regrids every sensor-driven AMR member; jp_bench reports the aggregate subcycle-weighted
  * throughput. Members are reachable with jp_ensemble_member for diagnostics and the test hooks. */
 int jp_ensemble_create(const jp_member* members, int count, int prec, double maxgb, int device, jp_ctx** out);
+/* Tile shape of the batched dense step (not in the reference; a tuning knob that jp_autotune also sets):
+ * set = 1 or 8 cells per lane (8 needs every member's lattice to be a multiple of 2048 cells), 0 to only
+ * query; *current receives the shape in effect. Every shape computes the same result. */
+int jp_ensemble_dense_shape(jp_ctx* ens, int set, int* current);
 /* Borrowed handle of member `index` (owned by the ensemble, valid until jp_destroy of the ensemble). */
 int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
 /* Autotuner (not in the reference; the reference has fixed launch shapes). Candidates are timed on the
  * current state; each timed kernel is re-run on unchanged inputs, so the state is unchanged.
- * - Ensemble: picks the workgroup count of every batched launch; result in groups[JP_ENS_KERNELS] (dense
- *   step, level-1 fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step, level-1 step
- *   + restriction). The two launches with a restriction write the parent level, so they are not timed:
+ * - Ensemble: picks the workgroup count of every batched launch, and the dense step's tile shape (1 or 8
+ *   cells per lane, jp_ensemble_dense_shape); result in groups[JP_ENS_KERNELS] (dense step, level-1
+ *   fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step, level-1 step +
+ *   restriction). The two launches with a restriction write the parent level, so they are not timed:
  *   they always run one workgroup per tile, which overlaps each tile's restriction tail.
  * - Uniform lattice: picks how steps run, as single steps or as fused pairs with one of the column shapes
  *   that divide the edge, and keeps the fastest. groups[0..3] receive the chosen column shape (x, y,

@@ -209,6 +209,13 @@ int jp_ensemble_create(const jp_member* members, int count, int prec, double max
   return ens_create(members, count, prec, maxgb, device, c);
 }
 
+int jp_ensemble_dense_shape(jp_ctx* ens, int set, int* current) {
+  if (!ens) return JP_ERR_ARG;
+  if (ens->kind != 2) return fail(ens, JP_ERR_STATE, "not an ensemble");
+  if (!ens->initialised) return fail(ens, JP_ERR_STATE, "jp_ensemble_dense_shape before jp_init");
+  return ens_dense_shape(ens, set, current);
+}
+
 int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member) {
   if (!ens || !member) return JP_ERR_ARG;
   if (ens->kind != 2) return fail(ens, JP_ERR_STATE, "not an ensemble");

@@ -120,6 +120,7 @@ bool ens_dynamic(const jp_ctx* c);
 int ens_adapt(const jp_ctx* c);
 double ens_work(const jp_ctx* c);
 int ens_autotune(jp_ctx* c, int* out);
+int ens_dense_shape(jp_ctx* c, int set, int* current);  // the dense step's cells per lane (1 or 8)
 void ens_destroy(jp_ctx* c);
 void ens_graph_key(const jp_ctx* c, const void* key[3]);  // the batch's buffer parities
 jp_ctx* ens_member(jp_ctx* c, int i);

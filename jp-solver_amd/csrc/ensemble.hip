@@ -27,6 +27,19 @@ __global__ __launch_bounds__(256) void kb_dense_step(const MemberDesc* __restric
                        uint32_t(w.y) * 256 + threadIdx.x);
   }
 }
+// items: (member, 2048-cell chunk): 8 consecutive x-cells per lane, one vector load per direction (the
+// single-context FP32 step's body)
+template <class S>
+__global__ __launch_bounds__(256) void kb_dense_step8(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+                                                      int n, int p0) {
+  using R = typename S::real;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const MemberDesc& d = md[w.x];
+    dense_step_vec<S, 8, false, false>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
+                                       uint32_t(w.y) * 256 + threadIdx.x);
+  }
+}
 // items: (member, fine tile); p: parity of the parent level's current buffer
 template <class S, int kLevel>
 __global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
@@ -125,6 +138,13 @@ hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* i
       break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t ens_launch_dense8(int prec, const MemberDesc* md, const int2* items, int n, int groups, int pa,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((kb_dense_step8<S>), grid_of(n, groups), dim3(256), 0, st, md, items, n, pa));
   return hipGetLastError();
 }
 

@@ -31,5 +31,9 @@ static_assert(EK_COUNT == JP_ENS_KERNELS, "one tuned launch shape per batched ke
 // buffer read, pb: the parent level's parity (the fused restriction's target; EK_STEP2R / EK_STEP1R only).
 hipError_t ens_launch(int prec, EnsKernel k, const MemberDesc* md, const int2* items, int n, int groups, int pa, int pb,
                       hipStream_t st);
+// The dense step in its second tile shape: items (member, 2048-cell chunk), 8 consecutive x-cells per lane
+// (every member's cell count must be a multiple of 2048).
+constexpr int kDense8Chunk = 2048;
+hipError_t ens_launch_dense8(int prec, const MemberDesc* md, const int2* items, int n, int groups, int pa, hipStream_t st);
 
 }  // namespace jp

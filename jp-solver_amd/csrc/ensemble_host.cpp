@@ -23,6 +23,12 @@ struct EnsState {
   size_t cap[EK_COUNT] = {};
   int n[EK_COUNT] = {};
   int groups[EK_COUNT] = {};
+  // the dense step's second tile shape (8 cells per lane over 2048-cell chunks): its work list, whether
+  // every member admits it, whether the autotuner chose it, and its workgroup count
+  int2* items8 = nullptr;
+  size_t cap8 = 0;
+  int n8 = 0, groups8 = 0;
+  bool dense8_ok = false, dense8 = false;
   int p0 = 0, p1 = 0, p2 = 0;  // parity of the current buffer of levels 0, 1, 2
   std::vector<long> sig;       // member counts and pool generation at upload
   bool uploaded = false;
@@ -34,6 +40,7 @@ struct EnsState {
 // would serialise those tails; with one workgroup per item the dispatcher overlaps them with other
 // items. The other kernels take their autotuned count.
 static int ens_groups(const EnsState& e, int k) {
+  if (k == EK_DENSE && e.dense8) return e.groups8;
   return (k == EK_STEP1R || k == EK_STEP2R) ? std::max(1, e.n[k]) : e.groups[k];
 }
 
@@ -99,12 +106,16 @@ static std::vector<long> signature(const EnsState& e) {
 static int upload(jp_ctx* c) {
   EnsState& e = *c->ens;
   e.hmd.clear();
-  std::vector<int2> L[EK_COUNT];
+  std::vector<int2> L[EK_COUNT], L8;
+  e.dense8_ok = true;
   for (int i = 0; i < int(e.m.size()); ++i) {
     const jp_ctx* m = e.m[i];
     e.hmd.push_back(describe(m));
     const long chunks = m->d.cells / 256;
     for (long k = 0; k < chunks; ++k) L[EK_DENSE].push_back(make_int2(i, int(k)));
+    // whole 2048-cell chunks only: a partial chunk would read and write past the member's lattice
+    if (m->d.cells % kDense8Chunk) e.dense8_ok = false;
+    for (long k = 0; k < m->d.cells / kDense8Chunk; ++k) L8.push_back(make_int2(i, int(k)));
     if (m->kind != 1) continue;
     const AmrState& a = m->amr;
     if (a.nf1) {
@@ -133,6 +144,16 @@ static int upload(jp_ctx* c) {
     }
     if (!L[k].empty()) HCK(c, hipMemcpy(e.items[k], L[k].data(), L[k].size() * sizeof(int2), hipMemcpyHostToDevice));
   }
+  if (!e.dense8_ok) L8.clear();
+  e.n8 = int(L8.size());
+  e.dense8 = e.dense8 && e.dense8_ok;
+  if (L8.size() > e.cap8) {
+    ctx_free(c, e.items8);
+    e.items8 = nullptr;
+    e.cap8 = std::max<size_t>(L8.size(), 1) * 5 / 4 + 16;
+    if (int r = ctx_alloc_t(c, &e.items8, e.cap8, "work list")) return r;
+  }
+  if (!L8.empty()) HCK(c, hipMemcpy(e.items8, L8.data(), L8.size() * sizeof(int2), hipMemcpyHostToDevice));
   e.p0 = e.p1 = e.p2 = 0;
   e.sig = signature(e);
   e.uploaded = true;
@@ -174,7 +195,8 @@ int ens_step_once(jp_ctx* c) {
   auto L = [&](EnsKernel k, int pa, int pb) -> hipError_t {
     return ens_launch(c->prec, k, e.dmd, e.items[k], e.n[k], ens_groups(e, k), pa, pb, c->st);
   };
-  HCK(c, L(EK_DENSE, e.p0, 0));
+  if (e.dense8) HCK(c, ens_launch_dense8(c->prec, e.dmd, e.items8, e.n8, e.groups8, e.p0, c->st));
+  else HCK(c, L(EK_DENSE, e.p0, 0));
   swap_all(e, 0);
   HCK(c, L(EK_FILL1, e.p0, 0));
   for (int s1 = 0; s1 < 2; ++s1) {
@@ -239,6 +261,29 @@ int ens_autotune(jp_ctx* c, int* out) {
   hipEvent_t a, b;
   HCK(c, hipEventCreate(&a));
   HCK(c, hipEventCreate(&b));
+  // median-of-3 time of the fastest workgroup count for a launch over n items; launch(groups) enqueues it
+  auto tune = [&](int n, auto&& launch, float* best_ms, int* best_groups) -> int {
+    *best_ms = 1e30f;
+    *best_groups = cu * 4;
+    for (int g : cand) {
+      const int groups = std::min(n, g * cu);
+      float t[3];
+      for (int r = 0; r < 3; ++r) {
+        HCK(c, hipEventRecord(a, c->st));
+        HCK(c, launch(groups));
+        HCK(c, hipEventRecord(b, c->st));
+        HCK(c, hipEventSynchronize(b));
+        HCK(c, hipEventElapsedTime(&t[r], a, b));
+      }
+      std::sort(t, t + 3);
+      if (t[1] < *best_ms) {
+        *best_ms = t[1];
+        *best_groups = groups;
+      }
+      if (groups == n) break;
+    }
+    return JP_OK;
+  };
   for (int k = 0; k < EK_COUNT; ++k) {
     if (k == EK_STEP2R || k == EK_STEP1R) continue;  // they write the parent level: one workgroup per item
     if (e.n[k] == 0) {
@@ -246,33 +291,46 @@ int ens_autotune(jp_ctx* c, int* out) {
       continue;
     }
     const int pa = (k == EK_DENSE || k == EK_FILL1) ? e.p0 : (k == EK_STEP2) ? e.p2 : e.p1;
-    const int pb = 0;
-    float best = 1e30f;
-    int bestg = cu * 4;
-    for (int g : cand) {
-      const int groups = std::min(e.n[k], g * cu);
-      float t[3];
-      for (int r = 0; r < 3; ++r) {
-        HCK(c, hipEventRecord(a, c->st));
-        HCK(c, ens_launch(c->prec, EnsKernel(k), e.dmd, e.items[k], e.n[k], groups, pa, pb, c->st));
-        HCK(c, hipEventRecord(b, c->st));
-        HCK(c, hipEventSynchronize(b));
-        HCK(c, hipEventElapsedTime(&t[r], a, b));
+    float best;
+    if (int r = tune(e.n[k], [&](int groups) {
+          return ens_launch(c->prec, EnsKernel(k), e.dmd, e.items[k], e.n[k], groups, pa, 0, c->st);
+        }, &best, &e.groups[k]))
+      return r;
+    if (k == EK_DENSE) {  // the dense step's tile shape: 1 or 8 cells per lane, whichever is faster
+      e.dense8 = false;
+      float best8;
+      if (e.dense8_ok && e.n8 > 0) {
+        if (int r = tune(e.n8, [&](int groups) {
+              return ens_launch_dense8(c->prec, e.dmd, e.items8, e.n8, groups, pa, c->st);
+            }, &best8, &e.groups8))
+          return r;
+        e.dense8 = best8 < best;
       }
-      std::sort(t, t + 3);
-      if (t[1] < best) {
-        best = t[1];
-        bestg = groups;
-      }
-      if (groups == e.n[k]) break;
     }
-    e.groups[k] = bestg;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   drop_graph(c);
   if (out)
     for (int k = 0; k < EK_COUNT; ++k) out[k] = ens_groups(e, k);
+  return JP_OK;
+}
+
+// Sets (set = 1 or 8) or only reports (set = 0) the dense step's tile shape in cells per lane. 8 needs
+// every member's cell count to be a multiple of the 2048-cell chunk.
+int ens_dense_shape(jp_ctx* c, int set, int* current) {
+  EnsState& e = *c->ens;
+  if (int r = ens_check(c)) return r;
+  if (set == 8) {
+    if (!e.dense8_ok) return fail(c, JP_ERR_ARG, "8 cells per lane needs every member's lattice to be a multiple of 2048 cells");
+    e.dense8 = true;
+  } else if (set == 1) {
+    e.dense8 = false;
+  } else if (set != 0) {
+    return fail(c, JP_ERR_ARG, "the dense step takes 1 or 8 cells per lane");
+  }
+  if (set) drop_graph(c);
+  if (current) *current = e.dense8 ? 8 : 1;
   return JP_OK;
 }
 
@@ -292,6 +350,7 @@ int ens_create(const jp_member* mem, int count, int prec, double maxgb, int devi
   hipDeviceProp_t prop;
   HCK(c, hipGetDeviceProperties(&prop, device));
   for (int k = 0; k < EK_COUNT; ++k) e.groups[k] = prop.multiProcessorCount * 8;
+  e.groups8 = prop.multiProcessorCount * 8;
   for (int i = 0; i < count; ++i) {
     const jp_member& p = mem[i];
     jp_ctx* m = nullptr;

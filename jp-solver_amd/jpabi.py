@@ -115,7 +115,8 @@ def lib():
             "jp_get_counts": (i, [vp, ctypes.POINTER(Counts)]),
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
             "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]),
-            "jp_set_step_pairs": (i, [vp, i]), "jp_step_pair_shape": (i, [vp, vp]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
+            "jp_set_step_pairs": (i, [vp, i]), "jp_step_pair_shape": (i, [vp, vp]),
+            "jp_ensemble_dense_shape": (i, [vp, i, ctypes.POINTER(i)]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
             "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -394,3 +395,10 @@ class Ensemble(_Ctx):
         g = (ctypes.c_int * ENS_KERNELS)()
         self._check(lib().jp_autotune(self.h, g))
         return list(g)
+
+    def dense_shape(self, cells_per_lane=0):
+        """Sets (1 or 8) or with 0 only reads the batched dense step's cells per lane; returns the shape in
+        effect."""
+        cur = ctypes.c_int()
+        self._check(lib().jp_ensemble_dense_shape(self.h, cells_per_lane, ctypes.byref(cur)))
+        return cur.value

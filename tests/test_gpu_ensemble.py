@@ -69,6 +69,45 @@ def test_graph_replay_and_regrid_match():
         same(ens, alone)
 
 
+# members whose lattices split into whole 2048-cell chunks (edges that are multiples of 16): the dense step
+# can then run 8 cells per lane
+MEMBERS16 = [
+    {"n": 32, "re": 1600.0, "u0": 0.05, "amr": 0},
+    {"n": 16, "re": 900.0, "u0": 0.04, "amr": 0, "phase": (0.3, 1.1, 2.0)},
+    {"n": 16, "re": 1600.0, "u0": 0.06, "amr": 1, "l1": jpabi.MODE_SENSOR, "l2": jpabi.MODE_SENSOR, "frac1": 0.4,
+     "frac2": 0.3, "adapt": 3, "phase": (1.0, 0.5, 0.25)},
+]
+
+
+@pytest.mark.parametrize("prec", [jpabi.PREC_FP32, jpabi.PREC_FP16])
+def test_dense_tile_shapes_match_single_contexts(prec):
+    """The batched dense step with 8 cells per lane and with 1 computes what the members compute alone."""
+    ens = jpabi.Ensemble(MEMBERS16, prec=prec, maxgb=16.0)
+    ens.init()
+    alone = [standalone(m, prec) for m in MEMBERS16]
+    for a in alone:
+        a.init()
+    assert ens.dense_shape(8) == 8
+    for shape in (8, 1, 8):
+        assert ens.dense_shape(shape) == shape
+        ens.step(3)
+        for a in alone:
+            a.step(3)
+        for m, e, a in zip(MEMBERS16, ens.members, alone):
+            for lvl in range(levels(m)):
+                assert np.array_equal(e.raw(lvl), a.raw(lvl)), (shape, m, lvl)
+
+
+def test_dense8_needs_whole_chunks():
+    """A member whose lattice is not a multiple of 2048 cells (24^3) keeps the dense step at 1 cell per lane."""
+    ens = jpabi.Ensemble(MEMBERS, maxgb=16.0)
+    ens.init()
+    with pytest.raises(jpabi.JPError) as e:
+        ens.dense_shape(8)
+    assert e.value.code == jpabi.JP_ERR_ARG
+    assert ens.dense_shape(0) == 1
+
+
 def test_autotune_keeps_state():
     ens = jpabi.Ensemble(MEMBERS, maxgb=16.0)
     ens.init()
