@@ -51,8 +51,10 @@ typedef struct {
   double phase[3]; /* TGV phase shift per axis in radians (0 = the reference's initial state) */
   double scale;    /* FP16/BF16 store scale; 0 = default 2^floor(log2(2048/u0)) */
   int device;      /* HIP device ordinal */
-  long plane_pad;  /* elements appended to each of the 19 population planes, a multiple of 64 (HBM
-                      channel spread); 0 = the library's default for the lattice, -1 = none */
+  long plane_pad;  /* device population layout: 0 = the library's default (z-layers, the 19 directions
+                      of one z-layer adjacent; direction planes for FP64); -1 = 19 direction planes; a
+                      positive multiple of 64 = direction planes with that many elements appended to
+                      each (HBM channel spread). Every layout computes the same result. */
 } jp_uniform_params;
 
 /* Derived run constants: src/tgv3d_uniform.cu:151-153 (uniform) and src/tgv3d_amr.cu:659-664 (AMR). */

@@ -164,7 +164,7 @@ template <class S> __device__ inline void prolong_l0(const typename S::raw* __re
     const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
     const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
     R fq[Q], nq[Q];
-    load_dense<S>(c0, G.g0.plane, dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0)), dec, fq);
+    load_dense<S>(c0, G.g0.qs, dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0)), dec, fq);
     const Mom<R> m = moments(fq);
     const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
     sfor<Q>([&](auto qc) {
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(512) void k_sensor(const typename S::raw* __restric
   __shared__ float sm[8];
   for_tiles<kLoop>(dn, [=](int b, int cell) {
     R v[Q];
-    if constexpr (kDense) load_dense<S>(f, G.g0.plane, dense_of_tile(G, b, cell), dec, v);
+    if constexpr (kDense) load_dense<S>(f, G.g0.qs, dense_of_tile(G, b, cell), dec, v);
     else load_tile<S>(f, b, cell, dec, v);
     const Mom<R> m = moments(v);
     const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_composite(const typename S::raw* __rest
     const int g1i = gi / 2, g1j = gj / 2, g1k = gk / 2;
     load_tile<S>(f1, lb, ((g1k % MB) * MB + (g1j % MB)) * MB + (g1i % MB), dec1, v);
   } else {
-    load_dense<S>(c0, G.g0.plane, dense_cell(G, gi / 4, gj / 4, gk / 4), dec0, v);
+    load_dense<S>(c0, G.g0.qs, dense_cell(G, gi / 4, gj / 4, gk / 4), dec0, v);
   }
   const Mom<R> m = moments(v);
   V4<R> o;

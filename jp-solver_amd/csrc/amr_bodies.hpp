@@ -79,6 +79,7 @@ template <class S> __device__ __forceinline__ void load_tile(const typename S::r
     out[q] = S::template load<q>(f[tidx(b, q, cell)], dec);
   });
 }
+// idx: linear cell of AMR level 0, which is stored in direction planes (qs = plane, zs = N^2)
 template <class S> __device__ __forceinline__ void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
                                                      float dec, typename S::real* out) {
   sfor<Q>([&](auto qc) {
@@ -147,7 +148,7 @@ __device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], type
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
-    if constexpr (kToDense) cf[uint64_t(q) * G.g0.plane + di] = v;
+    if constexpr (kToDense) cf[uint64_t(q) * G.g0.qs + di] = v;
     else cf[tidx(pcb, q, pc)] = v;
   });
 }
@@ -298,7 +299,7 @@ __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
     R v[Q];
-    load_dense<S>(c0, G.g0.plane, dense_cell(G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0)), dec, v);
+    load_dense<S>(c0, G.g0.qs, dense_cell(G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0)), dec, v);
     prep_values(v, nqs);
     sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
   }

@@ -682,10 +682,10 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
     HCK(c, hipEventCreateWithFlags(&a.ev_join[i], hipEventDisableTiming));
   }
   // pools
-  c->g0 = make_dense(N, 0);
+  c->g0 = make_dense(N, 0);  // direction planes: the interface kernels address level 0 by linear cell
   c->variant = default_variant(c->prec);
   const size_t e = prec_bytes(c->prec);
-  const size_t dense_b = size_t(Q) * c->g0.plane;
+  const size_t dense_b = size_t(c->g0.elems);
   if (int rc = ctx_alloc(c, &c->c0a, dense_b * e, "c0a")) return rc;
   if (int rc = ctx_alloc(c, &c->c0b, dense_b * e, "c0b")) return rc;
   if (int rc = alloc_l1_pool(c, true)) return rc;

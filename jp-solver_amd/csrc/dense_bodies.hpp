@@ -5,7 +5,8 @@
 
 namespace jp {
 
-// Source coordinate arrays for c in {-1, 0, +1} (index c + 1): source = destination - c, periodic.
+// Source coordinate arrays for c in {-1, 0, +1} (index c + 1): source = destination - c, periodic; zs are
+// offsets in an x-fastest N^3 field (the macro field), not in the population layout.
 struct Nbr {
   uint32_t xs[3], ys[3], zs[3];
 };
@@ -42,7 +43,7 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
   const uint32_t cell0 = lane_id * V;
   const uint32_t row = fdiv(cell0, g.divN), x0 = cell0 - row * N, z = fdiv(row, g.divN), y = row - z * N;
   const uint32_t ys[3] = {(y == N - 1 ? 0 : y + 1) * N, y * N, (y == 0 ? N - 1 : y - 1) * N};
-  const uint32_t zs[3] = {(z == N - 1 ? 0 : z + 1) * NN, z * NN, (z == 0 ? N - 1 : z - 1) * NN};
+  const uint64_t zs[3] = {uint64_t(z == N - 1 ? 0 : z + 1) * g.zs, uint64_t(z) * g.zs, uint64_t(z == 0 ? N - 1 : z - 1) * g.zs};
   const uint32_t xl = x0 == 0 ? N - 1 : x0 - 1, xr = x0 + V == N ? 0 : x0 + V;
   const int lane = threadIdx.x & 63;
   const bool left_edge = lane == 0 || x0 == 0, right_edge = lane == 63 || x0 + V == N;
@@ -50,7 +51,7 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int cx = CX(q);
-    const raw* base = fa + uint64_t(q) * g.plane + zs[CZ(q) + 1] + ys[CY(q) + 1];
+    const raw* base = fa + uint64_t(q) * g.qs + zs[CZ(q) + 1] + ys[CY(q) + 1];
     const vec* vp = reinterpret_cast<const vec*>(base + x0);
     vec w;
     if constexpr (kNtLoad) w = __builtin_nontemporal_load(vp);
@@ -90,7 +91,7 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
     vec w;
 #pragma unroll
     for (int j = 0; j < V; ++j) w[j] = S::template out<q>(f[q][j], enc);
-    vec* p = reinterpret_cast<vec*>(fb + uint64_t(q) * g.plane + cell0);
+    vec* p = reinterpret_cast<vec*>(fb + uint64_t(q) * g.qs + zs[1] + (cell0 - z * NN));
     if constexpr (kNtStore) __builtin_nontemporal_store(w, p);
     else *p = w;
   });
@@ -102,13 +103,13 @@ template <class S, bool kNtLoad = true, bool kNtStore = true>
 __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                        const Dense& g, typename S::real omega, float enc, float dec, uint32_t cell) {
   using R = typename S::real;
-  const uint32_t row = fdiv(cell, g.divN), x = cell - row * g.N, z = fdiv(row, g.divN), y = row - z * g.N;
-  const Nbr n = neighbours(x, y, z, g.N);
+  const uint32_t N = g.N, row = fdiv(cell, g.divN), x = cell - row * N, z = fdiv(row, g.divN), y = row - z * N;
+  const Nbr n = neighbours(x, y, z, N);
+  const uint64_t zs[3] = {uint64_t(z == N - 1 ? 0 : z + 1) * g.zs, uint64_t(z) * g.zs, uint64_t(z == 0 ? N - 1 : z - 1) * g.zs};
   R fin[Q];
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const uint32_t src = n.zs[CZ(q) + 1] + n.ys[CY(q) + 1] + n.xs[CX(q) + 1];
-    const typename S::raw* p = fa + uint64_t(q) * g.plane + src;
+    const typename S::raw* p = fa + uint64_t(q) * g.qs + zs[CZ(q) + 1] + (n.ys[CY(q) + 1] + n.xs[CX(q) + 1]);
     if constexpr (kNtLoad) fin[q] = S::template in<q>(__builtin_nontemporal_load(p), dec);
     else fin[q] = S::template in<q>(*p, dec);
   });
@@ -116,7 +117,7 @@ __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, t
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], omega), enc);
-    typename S::raw* p = fb + uint64_t(q) * g.plane + cell;
+    typename S::raw* p = fb + uint64_t(q) * g.qs + zs[1] + (y * N + x);
     if constexpr (kNtStore) __builtin_nontemporal_store(v, p);
     else *p = v;
   });

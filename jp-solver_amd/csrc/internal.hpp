@@ -1,6 +1,8 @@
 // Internal launch interface between the C-ABI layer (api.cpp) and the kernel translation units.
-// Populations of a dense periodic lattice (the uniform lattice and AMR level 0) are stored as 19
-// x-fastest planes: f[q*plane + (z*N + y)*N + x]. Scattered AMR tiles (levels 1 and 2) are stored
+// Populations of a dense periodic lattice (the uniform lattice and AMR level 0) sit at
+// f[q*qs + z*zs + y*N + x], x fastest, in one of two layouts: direction planes (qs = N^3 + pad, zs = N^2:
+// AMR level 0, and the uniform lattice on request) or z-layers (qs = N^2, zs = 19 N^2: the 19 directions
+// of one z-layer adjacent, the uniform default). Scattered AMR tiles (levels 1 and 2) are stored
 // tile-major: f[(tile*19 + q)*512 + cell], cell = (k*8 + j)*8 + i.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -14,13 +16,30 @@ namespace jp {
 
 struct Dense {
   uint32_t N;      // edge
-  uint64_t plane;  // elements between consecutive q planes (>= N^3)
+  uint64_t qs;     // elements between directions
+  uint64_t zs;     // elements between z-layers
+  uint64_t elems;  // elements of the buffer
   FastDiv divN;
+  // element of direction q at z-layer z, in-layer offset r = y*N + x
+  __host__ __device__ uint64_t at(int q, uint32_t z, uint32_t r) const { return uint64_t(q) * qs + uint64_t(z) * zs + r; }
 };
+// Direction planes of N^3 + pad elements.
 inline Dense make_dense(int N, uint64_t pad) {
   Dense d;
   d.N = uint32_t(N);
-  d.plane = uint64_t(N) * N * N + pad;
+  d.qs = uint64_t(N) * N * N + pad;
+  d.zs = uint64_t(N) * N;
+  d.elems = d.qs * Q;
+  d.divN = make_fastdiv(uint32_t(N));
+  return d;
+}
+// z-layers: the 19 directions of a layer adjacent.
+inline Dense make_dense_layers(int N) {
+  Dense d;
+  d.N = uint32_t(N);
+  d.qs = uint64_t(N) * N;
+  d.zs = d.qs * Q;
+  d.elems = d.zs * uint64_t(N);
   d.divN = make_fastdiv(uint32_t(N));
   return d;
 }

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(BS) void k_dense_init(typename S::raw* __restrict__
   const EqPre<R> e = eq_pre<R>(R(rho), R(ux), R(uy), R(0));
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    f[uint64_t(q) * g.plane + cell] = S::template store<q>(feq<q>(e), enc);
+    f[g.at(q, z, y * g.N + x)] = S::template store<q>(feq<q>(e), enc);
   });
 }
 
@@ -47,10 +47,11 @@ __global__ __launch_bounds__(BS) void k_dense_macro(const typename S::raw* __res
                                                     V4<typename S::real>* __restrict__ F) {
   using R = typename S::real;
   const uint32_t cell = blockIdx.x * BS + threadIdx.x;
+  const uint32_t z = fdiv(fdiv(cell, g.divN), g.divN), r = cell - z * g.N * g.N;
   R v[Q];
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    v[q] = S::template load<q>(f[uint64_t(q) * g.plane + cell], dec);
+    v[q] = S::template load<q>(f[g.at(q, z, r)], dec);
   });
   const Mom<R> m = moments(v);
   V4<R> o;
@@ -163,7 +164,7 @@ __global__ void k_dense_tiles(const T* __restrict__ src, T* __restrict__ dst, De
   const uint32_t b = blockIdx.x / Q, q = blockIdx.x % Q, cell = threadIdx.x;
   const uint32_t gx = g.N / MB, bx = b % gx, by = (b / gx) % gx, bz = b / (gx * gx);
   const uint32_t i = cell % MB, j = (cell / MB) % MB, k = cell / (MB * MB);
-  const uint64_t d = uint64_t(q) * g.plane + ((uint64_t(bz * MB + k) * g.N + (by * MB + j)) * g.N + (bx * MB + i));
+  const uint64_t d = g.at(int(q), bz * MB + k, (by * MB + j) * g.N + (bx * MB + i));
   const uint64_t t = (uint64_t(b) * Q + q) * MB3 + cell;
   if (kToTiles) dst[t] = src[d];
   else dst[d] = src[t];
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   constexpr int EX = TX + 2, EY = TY + 2, E = EX * EY, T = TX * TY;
   constexpr int PA = (E + BS - 1) / BS, PB = (T + BS - 1) / BS;
   __shared__ raw ring[kRingPlanes * T];
-  const uint32_t N = g.N, NN = N * N;
+  const uint32_t N = g.N;
   const uint32_t ntx = N / TX, nty = N / TY, nt = ntx * nty * (N / TZ);
   // every XCD streams a contiguous run of columns, so halo rows shared by neighbouring columns are read
   // through one L2
@@ -277,14 +278,14 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   auto load_plane = [&](raw(&r)[PA][Q], int it) {
     const int gp = int(zo0) - 1 + it;
     const uint32_t z = gp < 0 ? N - 1 : (gp >= int(N) ? 0 : gp);
-    const uint32_t zs[3] = {(z == 0 ? N - 1 : z - 1) * NN, z * NN, (z == N - 1 ? 0 : z + 1) * NN};
+    const uint64_t zs[3] = {uint64_t(z == 0 ? N - 1 : z - 1) * g.zs, uint64_t(z) * g.zs, uint64_t(z == N - 1 ? 0 : z + 1) * g.zs};
 #pragma unroll
     for (int a = 0; a < PA; ++a)
       sfor<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         // workgroup-uniform base of the source plane (scalar registers) plus the lane's 32-bit byte offset
         // within it: the load takes the scalar-base address form and needs no 64-bit lane arithmetic
-        const char* base = reinterpret_cast<const char*>(fa + (uint64_t(q) * g.plane + zs[1 - CZ(q)]));
+        const char* base = reinterpret_cast<const char*>(fa + (uint64_t(q) * g.qs + zs[1 - CZ(q)]));
         r[a][q] = *reinterpret_cast<const raw*>(base + (coly[a][1 - CY(q)] + colx[a][1 - CX(q)]) * uint32_t(sizeof(raw)));
       });
   };
@@ -325,10 +326,10 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
             fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * T + c], dec);
           });
           Collider<S>(fin).post_all(fin, omega);
-          const uint64_t cell = (uint64_t(o) * N + by * TY + oy) * N + bx * TX + ox;
+          const uint32_t r = (by * TY + oy) * N + bx * TX + ox;
           sfor<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            fb[uint64_t(q) * g.plane + cell] = S::template out<q>(fin[q], enc);
+            fb[g.at(q, o, r)] = S::template out<q>(fin[q], enc);
           });
         }
       }
