@@ -194,7 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--quick-extras", action="store_true", help="shorter secondary workloads")
-    ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling)")
+    ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling): amr-dynamic, amr-dynamic-graph, amr-gold, ensemble, uniform, uniform-single, field, diag, amr-diag")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -212,6 +212,7 @@ def main():
               "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
               "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),
+              "uniform-single": lambda: bench_uniform(args.n, args.prec, args.steps, pairs=False),
               "field": lambda: bench_field(args.n, 3),
               "diag": lambda: bench_diag(args.n, 5),
               "amr-diag": lambda: bench_amr_diag(5)}[args.only]
@@ -259,6 +260,8 @@ def main():
         "config": {"workload": f"uniform D3Q19 BGK TGV {n}^3 {args.prec.upper()} populations, Re 1600, u0 0.05, "
                                f"{args.steps} timed steps", "n": n, "re": 1600.0, "u0": 0.05,
                    "precision": args.prec, "graph": bool(args.graph),
+                   # the library's default device layout for the form (jp_uniform_params.plane_pad = 0)
+                   "layout": "direction planes" if args.prec == "fp64" else "z-layers",
                    "launch": (f"two fused steps per launch, columns {shape[0]} x {shape[1]} x {shape[2]}, "
                               f"{shape[3]} lanes" if shape[0] else "one step per launch") + " (picked by jp_autotune)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
