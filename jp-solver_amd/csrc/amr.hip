@@ -12,7 +12,7 @@ namespace jp {
 
 // ---- per-context kernels over the bodies above ----
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                    const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                                    const int* __restrict__ dn, typename S::real om, float enc, float dec) {
   __shared__ int snb[kNbr];
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(512) void k_fine_step(const typename S::raw* __rest
 // Last fine substep with the restriction into the parent level fused in (ptile/poct: the parent tile
 // and octant of each fine tile, i.e. l2pl1/l2oct or l1pcb/l1oct).
 template <class S, bool kToDense, bool kLoop>
-__global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                      const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                                      const int* __restrict__ dn, typename S::real om, float enc, float dec,
                                                      typename S::raw* __restrict__ cf, const int* __restrict__ ptile,
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(512) void k_fine_step_r(const typename S::raw* __re
 
 // Ghost fills, one fine tile per workgroup (the parent box and prep values staged in LDS).
 template <class S, bool kLoop>
-__global__ __launch_bounds__(256) void k_fill_l1(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
+__global__ __launch_bounds__(256) JP_FULL_OCCUPANCY(S) void k_fill_l1(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                                  const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                                  const int* __restrict__ nb1, const uint32_t* __restrict__ slots,
                                                  const int* __restrict__ dn, AmrGeo G, float nqs, float dec) {
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restri
 // its tile's ghost record (written to gh2 for the pair's second substep, and kept in LDS) and steps the tile
 // with the ghost values read from LDS, one launch and one pass over the tile's neighbourhood less.
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) void k_fill_step_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fill_step_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                                       const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
                                                       const int* __restrict__ f1pos, const int* __restrict__ l1grid,
                                                       const int* __restrict__ nb2, const uint32_t* __restrict__ slots,
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(512) void k_fill_step_l2(const typename S::raw* __r
 //  - k_step_r_l2_fill_l1: the pair's second level-2 substep with the restriction into level 1 (w2
 //    workgroups) and the level-1 ghost fill from the new level 0.
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) void k_dense_fill_step_l2(const typename S::raw* __restrict__ c0a,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2(const typename S::raw* __restrict__ c0a,
                                                             typename S::raw* __restrict__ c0b, Dense g0,
                                                             typename S::real om0, float enc0, float dec0, int nd,
                                                             const typename S::raw* __restrict__ f1,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(512) void k_dense_fill_step_l2(const typename S::ra
   });
 }
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                            const typename S::real* __restrict__ gh2,
                                                            const int* __restrict__ nb2, int n2, const int* __restrict__ dn2,
                                                            int w2, typename S::real om2, float enc2, float dec2,

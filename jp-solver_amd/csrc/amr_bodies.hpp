@@ -5,6 +5,12 @@
 
 namespace jp {
 
+// Per-tile AMR kernels of the 4- and 2-byte forms ask for 8 waves per SIMD: their vector registers allow
+// it (64 or fewer), and without the request the scalar registers of the looping and batched forms
+// (100-106) held them at 7. FP64 asks for 4, which its 84-106 vector registers already give (at 8 it
+// would spill).
+#define JP_FULL_OCCUPANCY(S) __attribute__((amdgpu_waves_per_eu(sizeof(typename S::raw) == 8 ? 4 : 8)))
+
 __device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;

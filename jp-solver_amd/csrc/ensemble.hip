@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void kb_dense_step8(const MemberDesc* __restri
 }
 // items: (member, fine tile); p: parity of the parent level's current buffer
 template <class S, int kLevel>
-__global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
+__global__ __launch_bounds__(256) JP_FULL_OCCUPANCY(S) void kb_fill(const MemberDesc* __restrict__ md, const int2* __restrict__ items, int n,
                                                int p) {
   using R = typename S::real;
   __shared__ FillLds<R> L;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void kb_fill(const MemberDesc* __restrict__ md
 }
 // items: (member, fine tile)
 template <class S, int kLevel>
-__global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void kb_fine_step(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
                                                     int n, int p) {
   using R = typename S::real;
   __shared__ int snb[kNbr];
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(512) void kb_fine_step(const MemberDesc* __restrict
 // Second substep of a fine level with the restriction into its parent fused in (the single-context
 // k_fine_step_r): items (member, fine tile); pf: the fine level's parity, pc: the parent level's.
 template <class S, int kLevel>
-__global__ __launch_bounds__(512) void kb_fine_step_r(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
+__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void kb_fine_step_r(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
                                                       int n, int pf, int pc) {
   using R = typename S::real;
   __shared__ int snb[kNbr];
