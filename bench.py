@@ -30,8 +30,11 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBPS = 8000.0
 METRIC = "GLUPS + % HBM roofline, uniform & AMR TGV, FP32 vs FP16-store at matched err"
 # HBM bytes per launch of the 512^3 FP32 step: the fused two-step launch, and the single-step kernel
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json")
-TRAFFIC_FILE_SINGLE = os.path.join(ROOT, "profiles", "uniform512_fp32_single_step_traffic.json")
+# measured HBM bytes per launch of the 512^3 FP32 lattice, per launch form (column shape; zeros: single steps)
+TRAFFIC_FILES = {(128, 4, 128, 832): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json"),
+                 (64, 8, 128, 704): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_64x8.json"),
+                 (32, 16, 128, 640): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_32x16.json"),
+                 (0, 0, 0, 0): os.path.join(ROOT, "profiles", "uniform512_fp32_single_step_traffic.json")}
 
 
 def cpu_baseline(target_s=15.0):
@@ -49,12 +52,14 @@ def cpu_baseline(target_s=15.0):
                       f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
 
 
-def bench_uniform(n, prec, steps, warmup=20, pairs=True):
+def bench_uniform(n, prec, steps, warmup=20, pairs=True, pair_shape=-1):
     import jpabi
 
     g = jpabi.Uniform(n, re=1600.0, u0=0.05, prec=jpabi.PREC_NAMES[prec])
     g.init()
     g.set_step_pairs(pairs)
+    if pairs and pair_shape >= 0:
+        g.set_step_pair_shape(pair_shape)
     r = g.bench(steps, warmup=warmup)
     r["pair_shape"] = list(g.step_pair_shape())
     g.close()
@@ -194,6 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--quick-extras", action="store_true", help="shorter secondary workloads")
+    ap.add_argument("--pair-shape", type=int, default=-1,
+                    help="--only uniform: the fused pair's column shape (0: 128 x 4, 1: 64 x 8, 2: 32 x 16; -1: default)")
     ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling): amr-dynamic, amr-dynamic-graph, amr-gold, ensemble, uniform, uniform-single, field, diag, amr-diag")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -211,7 +218,7 @@ def main():
                                                      graph=True),
               "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
-              "uniform": lambda: bench_uniform(args.n, args.prec, args.steps),
+              "uniform": lambda: bench_uniform(args.n, args.prec, args.steps, pair_shape=args.pair_shape),
               "uniform-single": lambda: bench_uniform(args.n, args.prec, args.steps, pairs=False),
               "field": lambda: bench_field(args.n, 3),
               "diag": lambda: bench_diag(args.n, 5),
@@ -237,10 +244,8 @@ def main():
     # algorithmic bytes per launch / launch duration
     achieved = bpu * n ** 3 * per_launch / (ms_launch * 1e-3) / 1e9
     traffic = None
-    tfile = TRAFFIC_FILE if per_launch == 2 else TRAFFIC_FILE_SINGLE
-    # the counter summaries were taken with the 128 x 4 x 128 column and with single steps
-    profiled = shape == (0, 0, 0, 0) or tuple(shape) == (128, 4, 128, 832)
-    if os.path.exists(tfile) and args.prec == "fp32" and n == 512 and profiled:
+    tfile = TRAFFIC_FILES.get(tuple(shape))
+    if tfile and os.path.exists(tfile) and args.prec == "fp32" and n == 512:
         with open(tfile) as f:
             t = json.load(f)
         traffic = t.get("hbm_bytes_per_launch")

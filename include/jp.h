@@ -222,6 +222,11 @@ int jp_set_variant(jp_ctx* c, int variant);
  * bit-identical either way; an odd step count ends with a single step. Resets the column shape to the
  * first in preference order that divides N (jp_autotune may pick another). */
 int jp_set_step_pairs(jp_ctx* c, int on);
+/* Selects the fused pair's column shape by its index in the library's preference order (0: 128 x 4,
+ * 1: 64 x 8, 2: 32 x 16 cells; -1: the first that divides N) and turns the pairs on (not in the
+ * reference; a tuning knob, as jp_set_variant). JP_ERR_ARG when the shape does not divide N or the
+ * storage form does not fuse; JP_ERR_STATE on an AMR context. Every shape computes the same result. */
+int jp_set_step_pair_shape(jp_ctx* c, int pick);
 /* The fused pair's column shape {x, y, z extent, lanes per workgroup}; all zero when steps run singly. */
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]);
 

@@ -52,6 +52,14 @@ void ctx_free(jp_ctx* c, void* p) {
 // 58.2-61.9 GLUPS in direction planes, single steps even), FP16 (73.1-73.5 against 70.3-72.0) and BF16
 // (68.7 against 67.9); direction planes for FP64 (17.9-18.3 against 15.3-16.0).
 bool default_layers(int prec) { return prec != JP_PREC_FP64; }
+// Elements after each direction row of a z-layer. 32 where the lattice runs fused pairs (FP32 with a
+// column shape that divides N): at 512^3 the power-of-two row stride cost the pair 2.7 % (59.7 against
+// 61.3 GLUPS with the pad), while the single steps ran faster without it (FP32 34.9 against 34.3, FP16
+// 80.2 against 75.4 in the same process), so every other lattice keeps 0.
+uint64_t default_layer_pad(int prec, int N) {
+  int shape[4];
+  return step_pair_shape(prec, N, -1, shape) ? 32 : 0;
+}
 
 double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 / u0))); }
 // Dense-step launch variant per storage form, from the 512^3 sweep on MI355X: FP32 8 cells per lane in
@@ -135,7 +143,8 @@ int jp_uniform_create(const jp_uniform_params* p, jp_ctx** out) {
   if (p->plane_pad < -1 || (p->plane_pad > 0 && p->plane_pad % 64 != 0))
     return fail(c, JP_ERR_ARG, "plane_pad must be a non-negative multiple of 64, 0 (default) or -1 (none)");
   // 0: the default layout; -1 or a pad: direction planes with that many elements appended to each
-  c->g0 = p->plane_pad == 0 && default_layers(c->prec) ? make_dense_layers(N) : make_dense(N, p->plane_pad > 0 ? uint64_t(p->plane_pad) : 0);
+  c->g0 = p->plane_pad == 0 && default_layers(c->prec) ? make_dense_layers(N, default_layer_pad(c->prec, N))
+                                                        : make_dense(N, p->plane_pad > 0 ? uint64_t(p->plane_pad) : 0);
   const size_t bytes = size_t(c->g0.elems) * prec_bytes(c->prec);
   if (int r = ctx_alloc(c, &c->c0a, bytes, "populations a")) return r;
   if (int r = ctx_alloc(c, &c->c0b, bytes, "populations b")) return r;
@@ -385,6 +394,18 @@ int jp_set_step_pairs(jp_ctx* c, int on) {
   if (!c) return JP_ERR_ARG;
   c->step_pairs = on != 0;
   c->pair_pick = -1;
+  drop_graph(c);
+  return JP_OK;
+}
+
+int jp_set_step_pair_shape(jp_ctx* c, int pick) {
+  if (!c) return JP_ERR_ARG;
+  if (c->kind != 0) return fail(c, JP_ERR_STATE, "column shapes apply to a uniform context");
+  int shape[4];
+  if (pick < -1 || pick >= step_pair_shape_count() || !step_pair_shape(c->prec, c->d.n, pick, shape))
+    return fail(c, JP_ERR_ARG, "no such column shape for this lattice and storage form");
+  c->step_pairs = true;
+  c->pair_pick = pick;
   drop_graph(c);
   return JP_OK;
 }

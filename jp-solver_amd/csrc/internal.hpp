@@ -1,8 +1,8 @@
 // Internal launch interface between the C-ABI layer (api.cpp) and the kernel translation units.
 // Populations of a dense periodic lattice (the uniform lattice and AMR level 0) sit at
 // f[q*qs + z*zs + y*N + x], x fastest, in one of two layouts: direction planes (qs = N^3 + pad, zs = N^2:
-// AMR level 0, and the uniform lattice on request) or z-layers (qs = N^2, zs = 19 N^2: the 19 directions
-// of one z-layer adjacent, the uniform default). Scattered AMR tiles (levels 1 and 2) are stored
+// AMR level 0, and the uniform lattice on request) or z-layers (qs = N^2 + pad, zs = 19 qs: the 19
+// directions of one z-layer adjacent, the uniform default). Scattered AMR tiles (levels 1 and 2) are stored
 // tile-major: f[(tile*19 + q)*512 + cell], cell = (k*8 + j)*8 + i.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -33,11 +33,11 @@ inline Dense make_dense(int N, uint64_t pad) {
   d.divN = make_fastdiv(uint32_t(N));
   return d;
 }
-// z-layers: the 19 directions of a layer adjacent.
-inline Dense make_dense_layers(int N) {
+// z-layers: the 19 directions of a layer adjacent, each followed by pad elements (a multiple of 32).
+inline Dense make_dense_layers(int N, uint64_t pad = 0) {
   Dense d;
   d.N = uint32_t(N);
-  d.qs = uint64_t(N) * N;
+  d.qs = uint64_t(N) * N + pad;
   d.zs = d.qs * Q;
   d.elems = d.zs * uint64_t(N);
   d.divN = make_fastdiv(uint32_t(N));

@@ -116,6 +116,7 @@ def lib():
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
             "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]),
             "jp_set_step_pairs": (i, [vp, i]), "jp_step_pair_shape": (i, [vp, vp]),
+            "jp_set_step_pair_shape": (i, [vp, i]),
             "jp_ensemble_dense_shape": (i, [vp, i, ctypes.POINTER(i)]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
             "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
@@ -249,6 +250,11 @@ class _Ctx:
     def set_step_pairs(self, on=True):
         """Two fused steps per launch (default where a column shape applies) or single steps."""
         self._check(lib().jp_set_step_pairs(self.h, 1 if on else 0))
+
+    def set_step_pair_shape(self, pick):
+        """Fused pairs with column shape `pick` of the library's list (0: 128 x 4, 1: 64 x 8, 2: 32 x 16; -1:
+        the first that divides N)."""
+        self._check(lib().jp_set_step_pair_shape(self.h, pick))
 
     def step_pair_shape(self):
         """(x, y, z extent, lanes) of the fused pair's column; zeros when steps run singly."""

@@ -102,6 +102,38 @@ def test_fused_pairs_equal_single_steps(n, shape):
     assert np.array_equal(a.raw(), b.raw())
 
 
+def test_pair_shape_setter_selects_each_shape():
+    """jp_set_step_pair_shape selects each column shape that divides N; every shape gives the single
+    steps' result bit for bit; a shape that does not divide N, a form that does not fuse and an AMR
+    context are refused."""
+    n = 256
+    ref = jpabi.Uniform(n)
+    ref.init()
+    start = ref.raw()
+    ref.set_step_pairs(False)
+    ref.step(5)
+    expected = ref.raw()
+    ref.close()
+    a = jpabi.Uniform(n)
+    for pick, shape in enumerate([(128, 4, 128, 832), (64, 8, 128, 704), (32, 16, 128, 640)]):
+        a.set_step_pair_shape(pick)
+        assert a.step_pair_shape() == shape
+        a.set_raw(start)
+        a.step(5)  # two pairs and a single step
+        assert np.array_equal(a.raw(), expected), shape
+    a.set_step_pair_shape(-1)
+    assert a.step_pair_shape() == (128, 4, 128, 832)
+    a.close()
+    with pytest.raises(jpabi.JPError) as e:
+        jpabi.Uniform(96).set_step_pair_shape(0)  # 128 does not divide 96
+    assert e.value.code == jpabi.JP_ERR_ARG
+    with pytest.raises(jpabi.JPError):
+        jpabi.Uniform(64, prec=jpabi.PREC_FP16).set_step_pair_shape(0)
+    with pytest.raises(jpabi.JPError) as e:
+        jpabi.Amr(16, l1=jpabi.MODE_SLAB, maxgb=4.0).set_step_pair_shape(0)
+    assert e.value.code == jpabi.JP_ERR_STATE
+
+
 @pytest.mark.parametrize("n,prec", [(256, jpabi.PREC_FP32), (96, jpabi.PREC_FP32), (64, jpabi.PREC_FP16)])
 def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
     """The uniform autotuner times single steps and the fused column shapes that divide N, leaves the state
