@@ -682,7 +682,10 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
     HCK(c, hipEventCreateWithFlags(&a.ev_join[i], hipEventDisableTiming));
   }
   // pools
-  c->g0 = make_dense(N, 0);  // direction planes: the interface kernels address level 0 by linear cell
+  // Level 0 in direction planes (the interface kernels address it by linear cell), 32 elements after each:
+  // at N0 = 128 the 8 MiB power-of-two plane stride cost the AMR coarse step 3 % (0.896 against 0.868 ms);
+  // at N0 = 96 the pad changes nothing measurable.
+  c->g0 = make_dense(N, 32);
   c->variant = default_variant(c->prec);
   const size_t e = prec_bytes(c->prec);
   const size_t dense_b = size_t(c->g0.elems);
