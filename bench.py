@@ -31,9 +31,9 @@ PEAK_HBM_GBPS = 8000.0
 METRIC = "GLUPS + % HBM roofline, uniform & AMR TGV, FP32 vs FP16-store at matched err"
 # HBM bytes per launch of the 512^3 FP32 step: the fused two-step launch, and the single-step kernel
 # measured HBM bytes per launch of the 512^3 FP32 lattice, per launch form (column shape; zeros: single steps)
-TRAFFIC_FILES = {(128, 4, 128, 832): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json"),
-                 (64, 8, 128, 704): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_64x8.json"),
-                 (32, 16, 128, 640): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_32x16.json"),
+TRAFFIC_FILES = {(128, 4, 256, 832): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json"),
+                 (64, 8, 256, 704): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_64x8.json"),
+                 (32, 16, 256, 640): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_32x16.json"),
                  (0, 0, 0, 0): os.path.join(ROOT, "profiles", "uniform512_fp32_single_step_traffic.json")}
 
 

@@ -81,12 +81,12 @@ def test_fused_step_pairs_are_bit_exact(prec, n):
         assert np.array_equal(g.raw(), o.raw()), k
 
 
-@pytest.mark.parametrize("n,shape", [(256, (128, 4, 128)), (192, (64, 8, 64)), (160, (32, 16, 32)),
-                                     (224, (32, 16, 32))])
+@pytest.mark.parametrize("n,shape", [(256, (128, 4, 256)), (384, (128, 4, 128)), (192, (64, 8, 64)),
+                                     (160, (32, 16, 32)), (224, (32, 16, 32))])
 def test_fused_pairs_equal_single_steps(n, shape):
-    """Every column shape and z-segment length (128 x 4 x 128 at 256^3, 64 x 8 x 64 at 192^3, 32 x 16 x 32
-    at 160^3 and 224^3): the fused pass equals two single-step launches, bit for bit, also through graph
-    replays."""
+    """Every column shape and z-segment length (128 x 4 x 256 at 256^3, 128 x 4 x 128 at 384^3, 64 x 8 x 64
+    at 192^3, 32 x 16 x 32 at 160^3 and 224^3): the fused pass equals two single-step launches, bit for bit,
+    also through graph replays."""
     a = jpabi.Uniform(n)
     b = jpabi.Uniform(n)
     a.init()
@@ -115,14 +115,14 @@ def test_pair_shape_setter_selects_each_shape():
     expected = ref.raw()
     ref.close()
     a = jpabi.Uniform(n)
-    for pick, shape in enumerate([(128, 4, 128, 832), (64, 8, 128, 704), (32, 16, 128, 640)]):
+    for pick, shape in enumerate([(128, 4, 256, 832), (64, 8, 256, 704), (32, 16, 256, 640)]):
         a.set_step_pair_shape(pick)
         assert a.step_pair_shape() == shape
         a.set_raw(start)
         a.step(5)  # two pairs and a single step
         assert np.array_equal(a.raw(), expected), shape
     a.set_step_pair_shape(-1)
-    assert a.step_pair_shape() == (128, 4, 128, 832)
+    assert a.step_pair_shape() == (128, 4, 256, 832)
     a.close()
     with pytest.raises(jpabi.JPError) as e:
         jpabi.Uniform(96).set_step_pair_shape(0)  # 128 does not divide 96
@@ -147,9 +147,9 @@ def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
     assert g.step_pair_shape() == shape
     valid = {(0, 0, 0, 0)}
     if prec == jpabi.PREC_FP32:
-        valid |= {s for s in [(128, 4, 128, 832), (64, 8, 64, 704), (32, 16, 32, 640), (128, 4, 64, 832),
-                              (64, 8, 128, 704), (32, 16, 128, 640), (32, 16, 64, 640)]
-                  if n % s[0] == 0 and n % s[1] == 0 and n % s[2] == 0}
+        tz = next(t for t in (256, 128, 64, 32) if n % t == 0)  # the longest z-segment that divides n
+        valid |= {(x, y, tz, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
+                  if n % x == 0 and n % y == 0}
     assert shape in valid
     o.step(5)
     g.step(5)
