@@ -223,7 +223,9 @@ int jp_set_variant(jp_ctx* c, int variant);
  * first in preference order that divides N (jp_autotune may pick another). */
 int jp_set_step_pairs(jp_ctx* c, int on);
 /* Selects the fused pair's column shape by its index in the library's preference order (0: 128 x 4,
- * 1: 64 x 8, 2: 32 x 16 cells; -1: the first that divides N) and turns the pairs on (not in the
+ * 1: 64 x 8, 2: 32 x 16 cells, each with the longest z-segment of 256, 128, 64 or 32 planes that divides
+ * N; 3-5: the same columns with half that segment, at least 32 planes; -1: the first that applies) and
+ * turns the pairs on (not in the
  * reference; a tuning knob, as jp_set_variant). JP_ERR_ARG when the shape does not divide N or the
  * storage form does not fuse; JP_ERR_STATE on an AMR context. Every shape computes the same result. */
 int jp_set_step_pair_shape(jp_ctx* c, int pick);

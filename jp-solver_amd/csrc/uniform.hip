@@ -356,16 +356,20 @@ struct PairShape {
   int tx, ty, bs;
 };
 static const PairShape kPairShapes[] = {{128, 4, 832}, {64, 8, 704}, {32, 16, 640}};
-int step_pair_shape_count() { return int(sizeof(kPairShapes) / sizeof(kPairShapes[0])); }
+constexpr int kShapeCount = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
+// Launch forms 0 .. kShapeCount - 1: each column shape with the longest z-segment; the next kShapeCount: the
+// same with half that segment (at least 32 planes), for the autotuner to try where boxes differ.
+int step_pair_shape_count() { return 2 * kShapeCount; }
 bool step_pair_shape(int prec, int N, int pick, int* out4) {
   if (prec != 0) return false;
   // z-segment: the longest of 256, 128, 64, 32 planes that divides N (at 512^3 on padded z-layers 256 planes
   // ran 62.0 against 61.2 GLUPS for 128: two halo planes per segment are 0.8 % of the reads instead of 1.6 %)
-  const int tz = N % 256 == 0 ? 256 : (N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0)));
-  if (!tz) return false;
+  const int tz0 = N % 256 == 0 ? 256 : (N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0)));
+  if (!tz0) return false;
   for (int i = 0; i < step_pair_shape_count(); ++i) {
-    const PairShape& p = kPairShapes[i];
-    if ((pick >= 0 && i != pick) || N % p.tx || N % p.ty) continue;
+    const PairShape& p = kPairShapes[i % kShapeCount];
+    const int tz = i < kShapeCount ? tz0 : tz0 / 2;
+    if ((pick >= 0 && i != pick) || N % p.tx || N % p.ty || tz < 32) continue;
     out4[0] = p.tx;
     out4[1] = p.ty;
     out4[2] = tz;

@@ -252,8 +252,8 @@ class _Ctx:
         self._check(lib().jp_set_step_pairs(self.h, 1 if on else 0))
 
     def set_step_pair_shape(self, pick):
-        """Fused pairs with column shape `pick` of the library's list (0: 128 x 4, 1: 64 x 8, 2: 32 x 16; -1:
-        the first that divides N)."""
+        """Fused pairs with column shape `pick` of the library's list (0: 128 x 4, 1: 64 x 8, 2: 32 x 16 with the
+        longest z-segment that divides N; 3-5: the same with half that segment; -1: the first that applies)."""
         self._check(lib().jp_set_step_pair_shape(self.h, pick))
 
     def step_pair_shape(self):

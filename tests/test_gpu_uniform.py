@@ -115,7 +115,8 @@ def test_pair_shape_setter_selects_each_shape():
     expected = ref.raw()
     ref.close()
     a = jpabi.Uniform(n)
-    for pick, shape in enumerate([(128, 4, 256, 832), (64, 8, 256, 704), (32, 16, 256, 640)]):
+    for pick, shape in enumerate([(128, 4, 256, 832), (64, 8, 256, 704), (32, 16, 256, 640),
+                                  (128, 4, 128, 832), (64, 8, 128, 704), (32, 16, 128, 640)]):
         a.set_step_pair_shape(pick)
         assert a.step_pair_shape() == shape
         a.set_raw(start)
@@ -124,6 +125,8 @@ def test_pair_shape_setter_selects_each_shape():
     a.set_step_pair_shape(-1)
     assert a.step_pair_shape() == (128, 4, 256, 832)
     a.close()
+    with pytest.raises(jpabi.JPError):
+        jpabi.Uniform(32).set_step_pair_shape(3)  # a 16-plane segment is below the minimum
     with pytest.raises(jpabi.JPError) as e:
         jpabi.Uniform(96).set_step_pair_shape(0)  # 128 does not divide 96
     assert e.value.code == jpabi.JP_ERR_ARG
@@ -148,8 +151,8 @@ def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
     valid = {(0, 0, 0, 0)}
     if prec == jpabi.PREC_FP32:
         tz = next(t for t in (256, 128, 64, 32) if n % t == 0)  # the longest z-segment that divides n
-        valid |= {(x, y, tz, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
-                  if n % x == 0 and n % y == 0}
+        valid |= {(x, y, t, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
+                  for t in (tz, tz // 2) if n % x == 0 and n % y == 0 and t >= 32}
     assert shape in valid
     o.step(5)
     g.step(5)
