@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# ORACLE_LIB names another build of the same sources: the mutation tests load oracle copies with one
+# planted mistake through it, in a child process.
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 _lib = None
 
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP64 = 0, 1, 2, 3
@@ -54,6 +56,9 @@ def lib():
             "orc_amr_get_f64": (None, [vp, i, vp]), "orc_amr_params": (None, [vp, vp]),
             "orc_uni_qspeed": (None, [vp, vp, vp, vp]), "orc_amr_qspeed": (None, [vp, vp, vp]),
             "orc_qspeed_field": (None, [vp, i, i, vp, vp, vp]),
+            "orc_amr_run": (None, [vp, i]), "orc_amr_set_omega": (None, [vp, i, d]),
+            "orc_amr_ghost_elems": (l, [vp, i]), "orc_amr_get_ghost": (None, [vp, i, vp]),
+            "orc_amr_set_ghost": (None, [vp, i, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -221,6 +226,29 @@ class Amr:
         out = np.empty(n, dtype=np.float64)
         lib().orc_amr_get_f64(self.h, level, _ptr(out))
         return out
+
+    # single operations of the subcycle / regrid (oracle/capi.cpp AmrO::run)
+    OP_FILL_L1, OP_FILL_L2, OP_STEP_L1, OP_STEP_L2, OP_RESTRICT_L1, OP_RESTRICT_L2, OP_SEED_L1, OP_SEED_L2, \
+        OP_SENSOR = range(9)
+
+    def run(self, op):
+        lib().orc_amr_run(self.h, op)
+
+    def set_omega(self, level, w):
+        lib().orc_amr_set_omega(self.h, level, w)
+
+    def ghost(self, level):
+        """Ghost boxes of a fine level, (tiles, 19, 1000), box index (ek*10 + ej)*10 + ei."""
+        n = lib().orc_amr_ghost_elems(self.h, level)
+        out = np.empty(n, dtype=np.float64)
+        lib().orc_amr_get_ghost(self.h, level, _ptr(out))
+        return out.reshape(-1, 19, 1000)
+
+    def set_ghost(self, level, a):
+        n = lib().orc_amr_ghost_elems(self.h, level)
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        assert a.size == n
+        lib().orc_amr_set_ghost(self.h, level, _ptr(a))
 
     def params(self):
         out = np.zeros(12)

@@ -94,6 +94,11 @@ struct AmrBase {
   virtual void set_raw(int level, const void* src) = 0;
   virtual void get_f64(int level, double* dst) const = 0;
   virtual void params(double* out) const = 0;
+  virtual void run(int op) = 0;
+  virtual void set_omega(int level, double w) = 0;
+  virtual long ghost_elems(int level) const = 0;
+  virtual void get_ghost(int level, double* dst) const = 0;
+  virtual void set_ghost(int level, const double* src) = 0;
 };
 
 template <class S> struct AmrO : AmrBase {
@@ -180,6 +185,47 @@ template <class S> struct AmrO : AmrBase {
     double v[] = {a.nu, a.tau0, a.tau1, a.tau2, double(a.om0), double(a.om1), double(a.om2), a.nqs01, a.nqi10,
                   a.nqs12, a.nqi21, a.kf};
     for (int i = 0; i < 12; ++i) o[i] = v[i];
+  }
+  // One operation of the subcycle or the regrid on the current state, as base_step / regrid run it
+  // (src/tgv3d_amr.cu:825-836, 299-308, 382-393, 442-453). The interface pins drive each one alone.
+  void run(int op) override {
+    switch (op) {
+      case 0:  // prep over the level-0 skin + level-1 ghost fill (amr.cu:825-826)
+        a.prep_list(a.c0a, a.prep0, a.skin0, a.n_skin0, a.nqs01, a.P.sc[0].dec, true);
+        a.fillcf_L1p();
+        break;
+      case 1:  // prep over the level-1 skin + level-2 ghost fill (amr.cu:830-831)
+        a.prep_list(a.f1a, a.prep1, a.skin1, a.n_skin1, a.nqs12, a.P.sc[1].dec, false);
+        a.fillcf_L2p();
+        break;
+      case 2: a.step_L1(); break;
+      case 3: a.step_L2(); break;
+      case 4: a.restrict_level(a.c0a, a.f1a, a.rl0, a.fof0, a.nref0, a.nqi10, a.P.sc[0], a.P.sc[1]); break;
+      case 5: a.restrict_level(a.f1a, a.f2a, a.rl1, a.fof1, a.nref1, a.nqi21, a.P.sc[1], a.P.sc[2]); break;
+      case 6: a.seed_L1(); break;
+      case 7: a.seed_L2(); break;
+      case 8:  // sensors of both levels into the sen0 / sen1 tables
+        a.sensor(a.c0a, a.cnb0, a.sen0, a.P.sc[0].dec);
+        a.sensor(a.f1a, a.nf1, a.sen1, a.P.sc[1].dec);
+        break;
+    }
+  }
+  void set_omega(int level, double w) override {
+    using R = typename S::real;
+    (level == 0 ? a.om0 : (level == 1 ? a.om1 : a.om2)) = R(w);
+  }
+  // Ghost boxes in the reference's layout gh[(tile*19 + q)*1000 + (ek*10 + ej)*10 + ei] (amr.cu:179-185).
+  long ghost_elems(int level) const override { return long(level == 1 ? a.nf1 : a.nf2) * Q * EB3; }
+  void get_ghost(int level, double* dst) const override {
+    const auto& g = level == 1 ? a.gh1 : a.gh2;
+    long n = ghost_elems(level);
+    for (long i = 0; i < n; ++i) dst[i] = double(g[i]);
+  }
+  void set_ghost(int level, const double* src) override {
+    using R = typename S::real;
+    auto& g = level == 1 ? a.gh1 : a.gh2;
+    long n = ghost_elems(level);
+    for (long i = 0; i < n; ++i) g[i] = R(src[i]);
   }
 };
 
@@ -286,4 +332,10 @@ void orc_amr_get_raw(void* h, int level, void* dst) { static_cast<AmrBase*>(h)->
 void orc_amr_set_raw(void* h, int level, const void* src) { static_cast<AmrBase*>(h)->set_raw(level, src); }
 void orc_amr_get_f64(void* h, int level, double* dst) { static_cast<AmrBase*>(h)->get_f64(level, dst); }
 void orc_amr_params(void* h, double* out) { static_cast<AmrBase*>(h)->params(out); }
+// Interface test hooks: one subcycle/regrid operation alone, a level's relaxation rate, ghost boxes.
+void orc_amr_run(void* h, int op) { static_cast<AmrBase*>(h)->run(op); }
+void orc_amr_set_omega(void* h, int level, double w) { static_cast<AmrBase*>(h)->set_omega(level, w); }
+long orc_amr_ghost_elems(void* h, int level) { return static_cast<AmrBase*>(h)->ghost_elems(level); }
+void orc_amr_get_ghost(void* h, int level, double* dst) { static_cast<AmrBase*>(h)->get_ghost(level, dst); }
+void orc_amr_set_ghost(void* h, int level, const double* src) { static_cast<AmrBase*>(h)->set_ghost(level, src); }
 }

@@ -14,8 +14,12 @@
 //
 // Parity pins (see tests/test_oracle_*.py): closed forms of the initial diagnostics, conservation and
 // fixed-point identities of the collision, the omega = 0 pure-shift permutation, Hermite-projection
-// identities, trilinear partition of unity, brute-force table logic, std::sort / std::exclusive_scan /
-// stable std::copy_if semantics for the Thrust call sites, numpy's float16 rounding for the FP16 store.
+// identities, brute-force table logic, std::sort / std::exclusive_scan / stable std::copy_if semantics
+// for the Thrust call sites, numpy's float16 rounding for the FP16 store. The coarse-fine interface
+// (seeding, ghost fill, the fine steps' ghost reads, restriction, migration, sensor) is pinned by closed
+// forms on affine fields carrying a traceless Hermite mode, a renormalised constant, and an omega = 0
+// shift of coded values (tests/interface_pins.py); tests/test_oracle_mutants.py plants plausible mistakes
+// in copies of these sources and checks that the pins catch each one.
 #pragma once
 #include <algorithm>
 #include <cmath>
