@@ -4,18 +4,22 @@
 Workload: uniform periodic D3Q19 BGK Taylor-Green vortex, 512^3, FP32 populations, Re 1600, u0 0.05
 (two population buffers, 20.4 GB), the lattice's own closed-form initial state. A step is the pull
 stream-collide over the whole lattice; the library runs two fused steps per launch (k_dense_step2: each
-workgroup streams a 128 x 4 x 128 column and keeps the intermediate step in LDS), the launch form picked
-by the library's autotuner (jp_autotune) before the timed steps. W untimed steps, then
-K steps timed with HIP events on the context's stream (inside the library, jp_bench). value = GLUPS
-(1e9 cell updates / s).
+workgroup streams a column of 128 x 4, 64 x 8 or 32 x 16 cells over 256 or 128 z-planes and keeps the
+intermediate step in LDS), the launch form picked by the library's autotuner (jp_autotune) before the
+timed steps. W untimed steps, then K steps timed with HIP events on the context's stream (inside the
+library, jp_bench). value = GLUPS (1e9 cell updates / s).
 
-roofline: the algorithmic cost of a cell update is 19 x 4 B read + 19 x 4 B written = 152 B; achieved =
-152 B x 2 N^3 per launch / (average launch duration), against MI355X's 8 TB/s HBM3E peak. Because the
-intermediate step never leaves the chip, the fused launch moves fewer HBM bytes than that: traffic (HBM
-bytes per launch) comes from the committed rocprofv3 counter summary for this workload (profiles/),
-scaled by a copy kernel's counters, and extra.hbm_gbps_actual = traffic / launch duration.
+roofline: a launch must read the lattice once and write it once, whatever number of steps it fuses:
+19 x 4 B read + 19 x 4 B written = 152 B per cell per launch (the 152 B per cell update of a single step).
+achieved = 152 B x N^3 / (average launch duration), against MI355X's 8 TB/s HBM3E peak. The rate a
+single-step kernel would need for the same GLUPS (152 B per cell update) is reported separately as
+extra.effective_gbps_152B_per_update. traffic (measured HBM bytes per launch) comes from the committed
+rocprofv3 counter summary for the launch form the autotuner kept (profiles/), scaled by a copy kernel's
+counters; extra.hbm_gbps_actual = traffic / launch duration.
 cpu_baseline: the CPU oracle (the reference step restated in OpenMP C++) on the host cores, on a
-bounded sample of the same lattice update (128^3, a few tens of steps).
+bounded sample of the same lattice update (128^3, a few hundred steps); extra.cpu_baseline_amr times the
+oracle's AMR coarse step (static gold at coarse 32, the sensor-driven grid at coarse 48) with the
+reference's work formula.
 """
 import argparse
 import json
@@ -29,11 +33,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
 METRIC = "GLUPS + % HBM roofline, uniform & AMR TGV, FP32 vs FP16-store at matched err"
-# HBM bytes per launch of the 512^3 FP32 step: the fused two-step launch, and the single-step kernel
-# measured HBM bytes per launch of the 512^3 FP32 lattice, per launch form (column shape; zeros: single steps)
+# measured HBM bytes per launch of the 512^3 FP32 lattice, per launch form (x, y, z-segment, lanes of the
+# fused column; zeros: single steps)
 TRAFFIC_FILES = {(128, 4, 256, 832): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic.json"),
                  (64, 8, 256, 704): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_64x8.json"),
                  (32, 16, 256, 640): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_32x16.json"),
+                 (128, 4, 128, 832): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_128x4x128.json"),
+                 (64, 8, 128, 704): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_64x8x128.json"),
+                 (32, 16, 128, 640): os.path.join(ROOT, "profiles", "uniform512_fp32_traffic_32x16x128.json"),
                  (0, 0, 0, 0): os.path.join(ROOT, "profiles", "uniform512_fp32_single_step_traffic.json")}
 
 
@@ -50,6 +57,36 @@ def cpu_baseline(target_s=15.0):
     return {"value": glups, "unit": "GLUPS", "cores": oracle.lib().orc_threads(), "kind": "port",
             "sample": f"oracle uniform {n}^3 FP32 Re 1600, {steps} steps in {t:.1f} s "
                       f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
+
+
+def cpu_baseline_amr(target_s=8.0):
+    """The oracle's AMR coarse step on the host cores: static gold (coarse 32, --l1 all --l2 all) and the
+    sensor-driven grid of the dynamic case at coarse 48, the smallest coarse edge where its three levels all
+    exist (frac 0.3 / 0.1, regrid every 10 coarse steps).
+    Work per coarse step is (cnb0 + 2 nf1 + 4 nf2) x 512 cell updates (src/tgv3d_amr.cu:843)."""
+    import oracle
+
+    out = {"cores": oracle.lib().orc_threads()}
+    cases = {"amr_gold_32_128": dict(n=32, re=800.0, l1=oracle.MODE_ALL, l2=oracle.MODE_ALL, adapt=0),
+             "amr_dynamic_48_192": dict(n=48, re=1600.0, l1=oracle.MODE_SENSOR, l2=oracle.MODE_SENSOR, frac1=0.3,
+                                        frac2=0.1, adapt=10)}
+    for name, kw in cases.items():
+        adapt = kw.pop("adapt")
+        a = oracle.Amr(u0=0.05, **kw)
+        a.init()
+        work, steps, t = 0.0, 0, 0.0
+        while t < target_s and steps < 200:
+            c = a.counts()
+            t0 = time.time()
+            a.step(1)
+            if adapt and (steps + 1) % adapt == 0:
+                a.regrid()
+            t += time.time() - t0
+            work += (c["cnb0"] + 2 * c["nf1"] + 4 * c["nf2"]) * 512
+            steps += 1
+        out[name] = {"value": work / t / 1e9, "unit": "GLUPS", "coarse_steps": steps, "seconds": t,
+                     "counts": a.counts()}
+    return out
 
 
 def bench_uniform(n, prec, steps, warmup=20, pairs=True, pair_shape=-1):
@@ -200,8 +237,11 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--quick-extras", action="store_true", help="shorter secondary workloads")
     ap.add_argument("--pair-shape", type=int, default=-1,
-                    help="--only uniform: the fused pair's column shape (0: 128 x 4, 1: 64 x 8, 2: 32 x 16; -1: default)")
-    ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling): amr-dynamic, amr-dynamic-graph, amr-gold, ensemble, uniform, uniform-single, field, diag, amr-diag")
+                    help="--only uniform: the fused pair's column shape (0: 128 x 4, 1: 64 x 8, 2: 32 x 16, each "
+                         "over 256 z-planes; 3-5: the same over 128 z-planes; -1: default)")
+    ap.add_argument("--only", default="", help="run one secondary workload and print it (profiling): amr-dynamic, "
+                    "amr-dynamic-graph, amr-gold (these take --prec, and --n as the coarse edge), ensemble, uniform, "
+                    "uniform-single, field, diag, amr-diag")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -213,10 +253,13 @@ def main():
     if args.only:
         # a plain copy of known size in the same process calibrates the HBM byte counters
         jpabi.probe_copy(1 << 30, iters=2)
-        fn = {"amr-dynamic": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps),
-              "amr-dynamic-graph": lambda: bench_amr(96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps,
-                                                     graph=True),
-              "amr-gold": lambda: bench_amr(64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps, graph=True),
+        n0 = args.n if args.n != 512 else None
+        fn = {"amr-dynamic": lambda: bench_amr(n0 or 96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10, args.steps,
+                                               prec=args.prec),
+              "amr-dynamic-graph": lambda: bench_amr(n0 or 96, 1600.0, "sensor", "sensor", 0.3, 0.1, 10,
+                                                     args.steps, prec=args.prec, graph=True),
+              "amr-gold": lambda: bench_amr(n0 or 64, 800.0, "all", "all", 0.35, 0.35, 100, args.steps,
+                                            prec=args.prec, graph=True),
               "ensemble": lambda: bench_ensemble(args.steps),
               "uniform": lambda: bench_uniform(args.n, args.prec, args.steps, pair_shape=args.pair_shape),
               "uniform-single": lambda: bench_uniform(args.n, args.prec, args.steps, pairs=False),
@@ -241,14 +284,19 @@ def main():
     g.close()
     bpu = r["bytes_per_update"]
     ms_launch = r["ms_per_step"] * per_launch
-    # algorithmic bytes per launch / launch duration
-    achieved = bpu * n ** 3 * per_launch / (ms_launch * 1e-3) / 1e9
-    traffic = None
+    # compulsory bytes per launch: one read and one write of the lattice (152 B per cell in FP32)
+    achieved = bpu * n ** 3 / (ms_launch * 1e-3) / 1e9
+    # the rate a one-step-per-launch kernel would need for the same cell updates (152 B per update)
+    effective = bpu * n ** 3 * per_launch / (ms_launch * 1e-3) / 1e9
+    traffic, traffic_src = None, None
     tfile = TRAFFIC_FILES.get(tuple(shape))
-    if tfile and os.path.exists(tfile) and args.prec == "fp32" and n == 512:
-        with open(tfile) as f:
-            t = json.load(f)
-        traffic = t.get("hbm_bytes_per_launch")
+    if args.prec == "fp32" and n == 512:
+        if tfile and os.path.exists(tfile):
+            with open(tfile) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic_src = os.path.relpath(tfile, ROOT)
+        else:
+            traffic_src = f"no committed counter summary for launch form {tuple(shape)}"
     line = {
         "metric": METRIC,
         "value": r["glups"],
@@ -273,6 +321,11 @@ def main():
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
         "extra": {"copy_kernel_gbps": copy_gbps, "attainable_hbm_gbps": attainable,
                   "steps_per_launch": per_launch, "ms_per_launch": ms_launch,
+                  "compulsory_bytes_per_launch": bpu * n ** 3,
+                  "frac_of_attainable_copy": achieved / attainable["copy"],
+                  "effective_gbps_152B_per_update": effective,
+                  "frac_effective_152B_per_update": effective / PEAK_HBM_GBPS,
+                  "traffic_source": traffic_src,
                   "hbm_gbps_actual": traffic / (ms_launch * 1e-3) / 1e9 if traffic else None,
                   "frac_hbm_actual": traffic / (ms_launch * 1e-3) / 1e9 / PEAK_HBM_GBPS if traffic else None,
                   "frac_of_attainable_copy_actual": (traffic / (ms_launch * 1e-3) / 1e9 / attainable["copy"]
@@ -284,6 +337,7 @@ def main():
     if not args.no_cpu:
         t0 = time.time()
         line["cpu_baseline"] = cpu_baseline()
+        line["extra"]["cpu_baseline_amr"] = cpu_baseline_amr()
         line["extra"]["cpu_baseline_wall_s"] = time.time() - t0
     else:
         line["cpu_baseline"] = None

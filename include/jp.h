@@ -78,6 +78,8 @@ typedef struct {
   double gbps;          /* glups * bytes_per_update */
   double pool_gb;       /* device bytes held by the context */
   long steps;
+  double device_used_gb; /* device memory in use after the warm-up, total - free (hipMemGetInfo): the
+                            reference's "ctx+pools" figure (src/tgv3d_amr.cu:841, 861-862) */
 } jp_bench_result;
 
 /* Refinement modes of --l1 / --l2 (src/tgv3d_amr.cu:651, 777-797). */

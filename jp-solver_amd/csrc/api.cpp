@@ -600,6 +600,8 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
   HCK(c, hipStreamSynchronize(c->st));
   if (c->kind == 1)
     if (int r = amr_sync_counts(c)) return r;
+  size_t mem_free = 0, mem_total = 0;
+  HCK(c, hipMemGetInfo(&mem_free, &mem_total));
   // Work of a segment in cell updates: the level counts it ran with. After a device-resident regrid the
   // host does not know them, so each segment's counts are copied to pinned memory behind its steps and
   // summed after the timed loop.
@@ -652,6 +654,7 @@ int jp_bench(jp_ctx* c, long steps, long warmup, int use_graph, jp_bench_result*
   res->bytes_per_update = 38.0 * double(prec_bytes(c->prec));
   res->gbps = res->glups * res->bytes_per_update;
   res->pool_gb = double(c->pool_bytes) / double(1ull << 30);
+  res->device_used_gb = double(mem_total - mem_free) / double(1ull << 30);
   return JP_OK;
 }
 

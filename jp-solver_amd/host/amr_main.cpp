@@ -61,18 +61,21 @@ int main(int argc, char** argv) {
   jp_counts k;
   jp_get_counts(c, &k);
   const long steps = long(tstar / (p.u0 * d.kf) + 0.5);
-  std::printf("[jp-solver amr] 3-level 3D. N0=%d N1=%d N2=%d Re=%g L1=%s(%ld blk) L2=%s(%ld blk) tau0=%.4f steps=%ld\n", p.n,
-              2 * p.n, 4 * p.n, p.re, l1, k.nf1, l2, k.nf2, d.tau0, steps);
+  // src/tgv3d_amr.cu:818-819
+  std::printf("[jp-solver amr] 3-level 3D. N0=%d N1=%d N2=%d Re=%g L1=%s(%ld blk) L2=%s(%ld blk) tau0=%.4f steps=%ld%s\n",
+              p.n, 2 * p.n, 4 * p.n, p.re, l1, k.nf1, l2, k.nf2, d.tau0, steps, vtk ? " (vti+vtp)" : "");
   if (bench) {
     // src/tgv3d_amr.cu:839-864: 20 warm-up steps, min(steps, 600) timed steps
     const long bsteps = argi(argc, argv, "--bench-steps", int(std::min(steps, 600L)));
     jp_bench_result r;
     if (int rc = jp_bench(c, bsteps, 20, graph ? 1 : 0, &r)) return die(c, rc);
     jp_get_counts(c, &k);
-    std::printf("[bench%s] AMR N0=%d(fine %d): %.3f s  work=%.3e updates  %.1f MLUPS(work)  pools %.3f GB  (L1=%ld L2=%ld)  "
-                "%.1f GB/s at %.0f B/update (%.1f%% of 8 TB/s HBM)\n",
-                graph ? "+graph" : "", p.n, 4 * p.n, r.ms_total / 1e3, r.work, r.glups * 1e3, r.pool_gb, k.nf1, k.nf2, r.gbps,
-                r.bytes_per_update, r.gbps / 80.0);
+    // src/tgv3d_amr.cu:861-862, then the roofline figures on a line of their own
+    std::printf("[bench%s] AMR N0=%d(fine %d): %.3f s  work=%.3e updates  %.1f MLUPS(work)  pools %.3f GB  ctx+pools %.3f GB  "
+                "(L1=%ld L2=%ld)\n",
+                graph ? "+graph" : "", p.n, 4 * p.n, r.ms_total / 1e3, r.work, r.glups * 1e3, r.pool_gb, r.device_used_gb,
+                k.nf1, k.nf2);
+    std::printf("  %.1f GB/s at %.0f B/update (%.1f%% of 8 TB/s HBM)\n", r.gbps, r.bytes_per_update, r.gbps / 80.0);
     jp_destroy(c);
     return 0;
   }
@@ -83,7 +86,7 @@ int main(int argc, char** argv) {
     if (int rc = jp_use_graph(c, 1)) return die(c, rc);
   const long sstep = graph ? 2 : 1;
   const long diagK = argi(argc, argv, "--diag-every", int(std::max(1L, steps / 160)));
-  const int frames = argi(argc, argv, "--frames", 160);  // src/tgv3d_amr.cu:869
+  const int frames = argi(argc, argv, "--frames", 240);  // src/tgv3d_amr.cu:652
   const long frameK = vtk ? std::max(1L, steps / std::max(1, frames)) : 0;
   int fi = 0;
   double E0 = -1, peak_ens = -1, peak_ts = -1, prevE = 1e30, mass0 = -1, mass_drift = 0;

@@ -75,7 +75,7 @@ class Derived(ctypes.Structure):
 class BenchResult(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("ms_per_step", ctypes.c_double), ("work", ctypes.c_double),
                 ("glups", ctypes.c_double), ("bytes_per_update", ctypes.c_double), ("gbps", ctypes.c_double),
-                ("pool_gb", ctypes.c_double), ("steps", ctypes.c_long)]
+                ("pool_gb", ctypes.c_double), ("steps", ctypes.c_long), ("device_used_gb", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

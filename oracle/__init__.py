@@ -152,13 +152,16 @@ class Amr:
     """3-level block AMR (src/tgv3d_amr.cu:643-837) on the CPU."""
 
     def __init__(self, n=32, re=800.0, u0=0.05, l1=MODE_ALL, l2=MODE_ALL, frac1=0.35, frac2=0.35,
-                 prec=PREC_FP32, budget_bytes=1e30, skipprep=0, skiprestr=0, phase=(0, 0, 0), scales=None):
+                 prec=PREC_FP32, budget_bytes=1e30, skipprep=0, skiprestr=0, phase=(0, 0, 0), scales=None,
+                 develop=300):
+        """develop: hidden level-0 steps of a sensor-mode initial build (the reference's 300); a test that
+        copies a developed state in from elsewhere can skip them."""
         self.n, self.prec = n, prec
         if scales is None:
             s = default_scale(u0) if prec in (1, 2) else 1.0
             scales = (s, s, s)
         self.scales = tuple(float(x) for x in scales)
-        ip = np.array([n, l1, l2, skipprep, skiprestr, prec], dtype=np.int32)
+        ip = np.array([n, l1, l2, skipprep, skiprestr, prec, develop], dtype=np.int32)
         dp = np.array([re, u0, frac1, frac2, budget_bytes, *phase, *self.scales], dtype=np.float64)
         self.h = lib().orc_amr_create(_ptr(ip), _ptr(dp))
         if not self.h:

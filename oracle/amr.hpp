@@ -20,6 +20,7 @@ struct AmrParams {
   double frac1 = 0.35, frac2 = 0.35;
   double budget_bytes = 1e30;  // fine-pool VRAM budget (src/tgv3d_amr.cu:684-688)
   int skipprep = 0, skiprestr = 0;
+  int develop = 300;  // hidden level-0 steps before a sensor-mode initial build (src/tgv3d_amr.cu:779)
   Scale sc[3];
 };
 
@@ -598,7 +599,7 @@ template <class S> struct Amr {
     else if (P.l1_mode == MODE_SLAB)
       for (int cb = 0; cb < cnb0; ++cb) ref0[cb] = (cb / (cgx * cgx) < cgx / 2) ? 1 : 0;
     else {
-      develop(300);
+      develop(P.develop);
       sensor(c0a, cnb0, sen0, P.sc[0].dec);
       if (sen0_in) std::copy(sen0_in, sen0_in + cnb0, sen0.begin());
       int tgt0 = std::min(cap_nref0, std::max(1, int(P.frac1 * cnb0)));

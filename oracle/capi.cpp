@@ -292,7 +292,7 @@ double orc_uni_time_steps(void* h, long n) {
 }
 
 // ---- AMR context ----
-// ip: n, l1_mode, l2_mode, skipprep, skiprestr, prec
+// ip: n, l1_mode, l2_mode, skipprep, skiprestr, prec, develop
 // dp: re, u0, frac1, frac2, budget_bytes, phase[3], scale[3]
 void* orc_amr_create(const int* ip, const double* dp) {
   AmrParams p;
@@ -301,6 +301,7 @@ void* orc_amr_create(const int* ip, const double* dp) {
   p.l2_mode = ip[2];
   p.skipprep = ip[3];
   p.skiprestr = ip[4];
+  p.develop = ip[6];
   p.re = dp[0];
   p.u0 = dp[1];
   p.frac1 = dp[2];

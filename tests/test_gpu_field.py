@@ -124,7 +124,7 @@ def test_level_boxes_follow_the_tables(tmp_path):
 
 
 def test_drivers_write_frames(tmp_path):
-    # src/tgv3d_uniform.cu:165, 193-196 and src/tgv3d_amr.cu:869, 889-895: frame every steps/frames steps
+    # src/tgv3d_uniform.cu:165, 193-196 and src/tgv3d_amr.cu:652, 869, 889-895: frame every steps/frames steps
     r = subprocess.run([os.path.join(BUILD, "uniform"), "--n", "16", "--tstar", "0.3", "--vtk", str(tmp_path / "u"),
                         "--frames", "3"], capture_output=True, text=True, timeout=120)
     assert "wrote 3 vti frames" in r.stdout, r.stdout + r.stderr
