@@ -85,6 +85,19 @@ static int uniform_step_once(jp_ctx* c) {
   return JP_OK;
 }
 
+// Partial sums of the diagnostic reductions: `slots` x 3 doubles, followed by the 3 results.
+static int ensure_partials(jp_ctx* c, size_t slots) {
+  if (c->partial_slots >= slots) return JP_OK;
+  ctx_free(c, c->partial);
+  c->partial = nullptr;
+  c->partial_slots = 0;
+  void* p;
+  if (int r = ctx_alloc(c, &p, slots * 3 * 8 + 64, "diagnostic partials")) return r;
+  c->partial = static_cast<double*>(p);
+  c->partial_slots = slots;
+  return JP_OK;
+}
+
 static int ensure_diag_buffers(jp_ctx* c, int nfield) {
   const size_t fb = size_t(nfield) * nfield * nfield * 4 * (c->prec == JP_PREC_FP64 ? 8 : 4);
   if (c->field_bytes < fb) {
@@ -94,12 +107,7 @@ static int ensure_diag_buffers(jp_ctx* c, int nfield) {
     if (int r = ctx_alloc(c, &c->field, fb, "diagnostic field")) return r;
     c->field_bytes = fb;
   }
-  if (!c->partial) {
-    void* p;
-    if (int r = ctx_alloc(c, &p, size_t(kDiagBlocks) * 3 * 8 + 64, "diagnostic partials")) return r;
-    c->partial = static_cast<double*>(p);
-  }
-  return JP_OK;
+  return ensure_partials(c, kDiagBlocks);
 }
 
 extern "C" {
@@ -435,16 +443,19 @@ int jp_diag(jp_ctx* c, double out[3]) {
   HCK(c, hipSetDevice(c->device));
   if (c->kind == 1)  // a diagnostic point also reports a device-resident regrid that overflowed a pool
     if (int r = amr_sync_counts(c)) return r;
-  const int nf = c->kind == 1 ? c->amr.N2 : c->d.n;
-  if (int r = ensure_diag_buffers(c, nf)) return r;
-  if (c->kind == 1) {
+  double* res;
+  if (c->kind == 1) {  // composite on the finest grid, then the reduction over it (src/tgv3d_amr.cu:871, 883)
+    const int nf = c->amr.N2;
+    if (int r = ensure_diag_buffers(c, nf)) return r;
     if (int r = amr_composite(c, c->field)) return r;
-  } else {
-    HCK(c, dense_macro(c->prec, c->c0a, c->g0, float(1.0 / c->d.scale[0]), c->field, c->st));
+    res = c->partial + c->partial_slots * 3;
+    HCK(c, field_diag(c->prec == JP_PREC_FP64, false, c->field, nf, c->partial, kDiagBlocks, res, c->st));
+  } else {  // one fused pass over the populations, no macroscopic field
+    if (int r = ensure_partials(c, dense_diag_items(c->d.n))) return r;
+    res = c->partial + c->partial_slots * 3;
+    HCK(c, dense_diag(c->prec, c->c0a, c->g0, float(1.0 / c->d.scale[0]), c->partial, res, c->st));
   }
-  HCK(c, field_diag(c->prec == JP_PREC_FP64, c->kind == 0, c->field, nf, c->partial, kDiagBlocks,
-                    c->partial + kDiagBlocks * 3, c->st));
-  HCK(c, hipMemcpyAsync(out, c->partial + kDiagBlocks * 3, 24, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipMemcpyAsync(out, res, 24, hipMemcpyDeviceToHost, c->st));
   HCK(c, hipStreamSynchronize(c->st));
   return JP_OK;
 }

@@ -66,7 +66,8 @@ struct jp_ctx {
   size_t out_bytes = 0;
   float* host_out = nullptr;  // pinned host copy of them for the frame writer
   size_t host_out_bytes = 0;
-  double* partial = nullptr;
+  double* partial = nullptr;  // diagnostic partial sums (3 doubles per slot), then the 3 results
+  size_t partial_slots = 0;
   // Captured pairs of steps, keyed by the buffers they start from (the ping-pong roles), so that an odd
   // step or a device-resident regrid selects another pair instead of recapturing. c0 holds the dense
   // pair's roles after the captured steps (two single steps restore them, a fused pair swaps them once),

@@ -81,6 +81,11 @@ hipError_t field_qspeed(bool dbl_field, bool diff_in_field, const void* field, i
 hipError_t field_diag(bool dbl_field, bool diff_in_field_type, const void* field, int N, double* partial,
                       int nblocks, double* out3, hipStream_t st);
 constexpr int kDiagBlocks = 1024;
+// E, enstrophy, mass of the dense lattice in one fused pass from the populations (no macro field): one
+// partial per work item (3 doubles each, dense_diag_items(N) of them) at `partial`, the three means at out3
+hipError_t dense_diag(int prec, const void* f, const Dense& g, float dec, double* partial, double* out3,
+                      hipStream_t st);
+uint32_t dense_diag_items(int N);
 hipError_t copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
 // mode 0 copy src -> dst, 1 read src, 2 write dst; bytes a multiple of 4096
 hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st);

@@ -137,6 +137,80 @@ __global__ __launch_bounds__(BS) void k_diag_final(const double* __restrict__ pa
   if (threadIdx.x < 3) out[threadIdx.x] = res[threadIdx.x] * inv;
 }
 
+// Energy, enstrophy and mass of the dense lattice in one pass, with no macroscopic field in HBM (the
+// uniform driver's diagnostics, src/tgv3d_uniform.cu:80-90, 99-108, 170-183). A work item is a column of
+// kDiagTX x kDiagTY cells over a segment of at most kDiagSeg z-planes. The workgroup computes rho and u of
+// each plane of the column's (kDiagTX + 2) x (kDiagTY + 2) extended tile, one plane ahead, into a ring of
+// three LDS planes; each lane accumulates its cell's terms in FP64 exactly as k_dense_macro +
+// k_field_diag<R, true> do (the same values, the same per-cell operation order). One partial per item,
+// stored at the item's index, so k_diag_final adds them in a fixed order: the sum is deterministic.
+// Items are dealt to the XCDs in contiguous runs (xcd_tile), so neighbouring columns share halo rows in
+// one L2.
+constexpr int kDiagTX = 64, kDiagTY = 8, kDiagSeg = 128, kDiagBS = kDiagTX * kDiagTY;
+template <class S>
+__global__ __launch_bounds__(kDiagBS) void k_dense_diag(const typename S::raw* __restrict__ f, Dense g, float dec,
+                                                       uint32_t ntx, uint32_t ncol, uint32_t nitems,
+                                                       double* __restrict__ partial) {
+  using R = typename S::real;
+  constexpr int EX = kDiagTX + 2, EY = kDiagTY + 2, NE = EX * EY;
+  __shared__ V4<R> ring[3][NE];
+  const uint32_t N = g.N;
+  const uint32_t item = uint32_t(xcd_tile(int(blockIdx.x), int(nitems)));
+  const uint32_t col = item % ncol, seg = item / ncol;
+  const uint32_t x0 = (col % ntx) * kDiagTX, y0 = (col / ntx) * kDiagTY;
+  const uint32_t z0 = seg * kDiagSeg, z1 = min(N, z0 + kDiagSeg);
+  // rho, u of the extended tile of plane z (x and y periodic)
+  auto plane = [&](V4<R>* P, uint32_t z) {
+    for (int e = threadIdx.x; e < NE; e += kDiagBS) {
+      const int ex = e % EX, ey = e / EX;
+      const uint32_t x = (x0 + ex + N - 1) % N, y = (y0 + ey + N - 1) % N, r = y * N + x;
+      R v[Q];
+      sfor<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        v[q] = S::template load<q>(f[g.at(q, z, r)], dec);
+      });
+      const Mom<R> m = moments(v);
+      V4<R> o;
+      o.a = m.rho;
+      o.b = m.ux / m.rho;
+      o.c = m.uy / m.rho;
+      o.d = m.uz / m.rho;
+      P[e] = o;
+    }
+  };
+  const int lx = threadIdx.x % kDiagTX, ly = threadIdx.x / kDiagTX;
+  const bool live = x0 + lx < N && y0 + ly < N;
+  const int c = (ly + 1) * EX + lx + 1;
+  double E = 0, ens = 0, mass = 0;
+  int sm = 0, sc = 1, sp = 2;  // ring slots of planes z - 1, z, z + 1
+  plane(ring[sm], z0 == 0 ? N - 1 : z0 - 1);
+  plane(ring[sc], z0);
+  for (uint32_t z = z0; z < z1; ++z) {
+    plane(ring[sp], z + 1 == N ? 0 : z + 1);
+    __syncthreads();
+    if (live) {
+      const V4<R>* P = ring[sc];
+      const V4<R> cc = P[c], xp = P[c + 1], xm = P[c - 1], yp = P[c + EX], ym = P[c - EX];
+      const V4<R> zp = ring[sp][c], zm = ring[sm][c];
+      auto dif = [](R p, R m) -> double { return double(p - m) * 0.5; };
+      const double dudy = dif(yp.b, ym.b), dudz = dif(zp.b, zm.b);
+      const double dvdx = dif(xp.c, xm.c), dvdz = dif(zp.c, zm.c);
+      const double dwdx = dif(xp.d, xm.d), dwdy = dif(yp.d, ym.d);
+      const double ox = dwdy - dvdz, oy = dudz - dwdx, oz = dvdx - dudy;
+      const double u = cc.b, v = cc.c, w = cc.d;
+      E += 0.5 * (u * u + v * v + w * w);
+      ens += 0.5 * (ox * ox + oy * oy + oz * oz);
+      mass += double(cc.a);
+    }
+    __syncthreads();
+    const int t = sm;
+    sm = sc;
+    sc = sp;
+    sp = t;
+  }
+  block_sum3<kDiagBS>(E, ens, mass, partial + 3 * item);
+}
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ void k_copy16(const u32x4* __restrict__ s, u32x4* __restrict__ d, size_t n) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
@@ -494,6 +568,37 @@ hipError_t field_diag(bool dbl, bool diff_in_field, const void* field, int N, do
   const double inv = 1.0 / (double(N) * N * N);
   hipLaunchKernelGGL((k_diag_final<kBS>), dim3(1), dim3(kBS), 0, st, partial, nblocks, inv, out3);
   return hipGetLastError();
+}
+
+static void diag_grid(uint32_t N, uint32_t* ntx, uint32_t* ncol, uint32_t* nitems) {
+  *ntx = (N + kDiagTX - 1) / kDiagTX;
+  *ncol = *ntx * ((N + kDiagTY - 1) / kDiagTY);
+  *nitems = *ncol * ((N + kDiagSeg - 1) / kDiagSeg);
+}
+uint32_t dense_diag_items(int N) {
+  uint32_t a, b, n;
+  diag_grid(uint32_t(N), &a, &b, &n);
+  return n;
+}
+template <class S>
+static hipError_t ddiag_t(const void* f, const Dense& g, float dec, double* partial, double* out3, hipStream_t st) {
+  uint32_t ntx, ncol, nitems;
+  diag_grid(g.N, &ntx, &ncol, &nitems);
+  hipLaunchKernelGGL((k_dense_diag<S>), dim3(nitems), dim3(kDiagBS), 0, st, static_cast<const typename S::raw*>(f),
+                     g, dec, ntx, ncol, nitems, partial);
+  const double inv = 1.0 / (double(g.N) * g.N * g.N);
+  hipLaunchKernelGGL((k_diag_final<kBS>), dim3(1), dim3(kBS), 0, st, partial, int(nitems), inv, out3);
+  return hipGetLastError();
+}
+hipError_t dense_diag(int prec, const void* f, const Dense& g, float dec, double* partial, double* out3,
+                      hipStream_t st) {
+  switch (prec) {
+    case 0: return ddiag_t<FormF32>(f, g, dec, partial, out3, st);
+    case 1: return ddiag_t<FormF16>(f, g, dec, partial, out3, st);
+    case 2: return ddiag_t<FormBF16>(f, g, dec, partial, out3, st);
+    case 3: return ddiag_t<FormF64>(f, g, dec, partial, out3, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st) {

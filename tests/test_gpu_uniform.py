@@ -242,6 +242,20 @@ def test_diag_matches_oracle(prec):
     assert g.diag() == pytest.approx(o.diag(), rel=1e-12)
 
 
+@pytest.mark.parametrize("n", [8, 24, 40, 136, 264])
+def test_fused_diag_edges(n):
+    """The fused diagnostic pass tiles the lattice in 64 x 8 columns over 128-plane segments; edges that
+    leave partial columns, rows and segments (and lattices smaller than one column) give the oracle's
+    values."""
+    o = orc.Uniform(n, prec=jpabi.PREC_FP32)
+    o.init()
+    o.step(3)
+    g = jpabi.Uniform(n, prec=jpabi.PREC_FP32)
+    g.set_raw(o.raw())
+    assert g.diag() == pytest.approx(o.diag(), rel=1e-12)
+    g.close()
+
+
 def test_initial_diagnostics_closed_form_fp64():
     n = 64
     g = jpabi.Uniform(n, prec=jpabi.PREC_FP64)
