@@ -238,6 +238,16 @@ int jp_step_pair_shape(const jp_ctx* c, int shape[4]);
  * order; 24 bytes cross to the host (src/tgv3d_uniform.cu:170-183; src/tgv3d_amr.cu:882-885). */
 int jp_diag(jp_ctx* c, double out[3]);
 
+/* jp_diag for a uniform context without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The partial sums go to a fixed array of the library (768 KiB on
+ * every GPU it runs on, held until the process ends), which holds the partials of any lattice
+ * jp_uniform_create accepts; a lattice needing more returns JP_ERR_ARG. The result equals jp_diag's bit
+ * for bit (same kernels, same summation order). Every call on one GPU shares the array, so calls that
+ * overlap on one GPU (contexts on different host threads or streams) race. JP_ERR_STATE for AMR and
+ * ensemble contexts. */
+int jp_diag_noalloc(jp_ctx* c, double out[3]);
+
 /* Pure step loop timed with HIP events on the context's stream (src/tgv3d_amr.cu:839-864; the uniform
  * reference has no bench path). A sensor-driven AMR context regrids every `adapt` steps in warm-up and
  * timed loop. use_graph replays pairs of steps from captured hipGraphs; the AMR regrids then run on

@@ -460,6 +460,25 @@ int jp_diag(jp_ctx* c, double out[3]) {
   return JP_OK;
 }
 
+int jp_diag_noalloc(jp_ctx* c, double out[3]) {
+  if (!c || !out) return JP_ERR_ARG;
+  if (c->kind != 0) return fail(c, JP_ERR_STATE, "jp_diag_noalloc takes a uniform context (AMR, ensembles: jp_diag)");
+  if (!c->initialised) return fail(c, JP_ERR_STATE, "jp_diag_noalloc before jp_init");
+  HCK(c, hipSetDevice(c->device));
+  double* sp = nullptr;
+  size_t slots = 0;
+  HCK(c, noalloc_scratch(c->device, &sp, &slots));
+  const uint32_t items = dense_diag_items(c->d.n);
+  if (items > slots)
+    return fail(c, JP_ERR_ARG, "jp_diag_noalloc: the lattice needs " + std::to_string(items) +
+                                   " partial sums, the fixed scratch array holds " + std::to_string(slots));
+  double* res = sp + 3 * slots;
+  HCK(c, dense_diag(c->prec, c->c0a, c->g0, float(1.0 / c->d.scale[0]), sp, res, c->st));
+  HCK(c, hipMemcpyAsync(out, res, 24, hipMemcpyDeviceToHost, c->st));
+  HCK(c, hipStreamSynchronize(c->st));
+  return JP_OK;
+}
+
 long jp_field_edge(jp_ctx* c) {
   if (!c || c->kind == 2) return 0;
   return c->kind == 1 ? c->amr.N2 : c->d.n;

@@ -86,6 +86,9 @@ constexpr int kDiagBlocks = 1024;
 hipError_t dense_diag(int prec, const void* f, const Dense& g, float dec, double* partial, double* out3,
                       hipStream_t st);
 uint32_t dense_diag_items(int N);
+// The library's fixed per-device scratch array (no allocation): its device address on `device`, which must
+// be current, and how many partial slots it holds (the 3 results follow them).
+hipError_t noalloc_scratch(int device, double** ptr, size_t* slots);
 hipError_t copy_probe(const void* src, void* dst, size_t bytes, hipStream_t st);
 // mode 0 copy src -> dst, 1 read src, 2 write dst; bytes a multiple of 4096
 hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st);

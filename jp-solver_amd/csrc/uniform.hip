@@ -9,6 +9,7 @@
 // runs two steps per launch instead (k_dense_step2), which keeps the intermediate step on the chip and
 // moves fewer bytes than two single steps.
 #include <algorithm>
+#include <mutex>
 
 #include "dense_bodies.hpp"
 
@@ -599,6 +600,31 @@ hipError_t dense_diag(int prec, const void* f, const Dense& g, float dec, double
     case 3: return ddiag_t<FormF64>(f, g, dec, partial, out3, st);
   }
   return hipErrorInvalidValue;
+}
+
+// Fixed device array for the partial sums of dense_diag when the caller cannot allocate (jp_diag_noalloc):
+// kNoAllocSlots partials of 3 doubles, then the 3 results (768 KiB). It holds the partials of every lattice
+// the library creates (edge <= 1280: 32000 items). Each GPU the library runs on holds its own copy until the
+// process ends; every call on one GPU shares it. Host code must not pass `scratch` itself to a kernel (in
+// host code the name is a host-side stand-in): noalloc_scratch looks up each device's address once, with
+// that device current, and every launch takes that address.
+constexpr size_t kNoAllocSlots = 32768;
+static __device__ __align__(256) double scratch[3 * kNoAllocSlots + 8];
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static double* scratch_dev[kMaxDevices] = {};
+hipError_t noalloc_scratch(int device, double** ptr, size_t* slots) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // the caller made `device` current
+    if (e != hipSuccess) return e;
+    scratch_dev[device] = static_cast<double*>(p);
+  }
+  *ptr = scratch_dev[device];
+  *slots = kNoAllocSlots;
+  return hipSuccess;
 }
 
 hipError_t bandwidth_probe(int mode, const void* src, void* dst, size_t bytes, hipStream_t st) {

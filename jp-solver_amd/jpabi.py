@@ -100,6 +100,7 @@ def lib():
             "jp_version": (ctypes.c_char_p, []),
             "jp_uniform_create": (i, [ctypes.POINTER(UniformParams), ctypes.POINTER(vp)]),
             "jp_init": (i, [vp]), "jp_step": (i, [vp, l]), "jp_diag": (i, [vp, vp]),
+            "jp_diag_noalloc": (i, [vp, vp]),
             "jp_bench": (i, [vp, l, l, i, ctypes.POINTER(BenchResult)]),
             "jp_get_derived": (i, [vp, ctypes.POINTER(Derived)]),
             "jp_get_populations": (i, [vp, i, vp, sz]), "jp_set_populations": (i, [vp, i, vp, sz]),
@@ -190,6 +191,12 @@ class _Ctx:
     def diag(self):
         out = np.zeros(3)
         self._check(lib().jp_diag(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def diag_noalloc(self):
+        """jp_diag of a uniform lattice through the allocation-free entry point."""
+        out = np.zeros(3)
+        self._check(lib().jp_diag_noalloc(self.h, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
     def field(self, velocity=False):
