@@ -220,16 +220,17 @@ int jp_set_variant(jp_ctx* c, int variant);
  * k_step per step, src/tgv3d_uniform.cu:166-167; a tuning knob): a workgroup streams a column of the
  * lattice and keeps the intermediate step in LDS, so HBM sees one read and one write per two steps plus
  * the column halos. On by default for FP32 storage where a column shape divides N (a multiple of 32);
- * the 16-bit forms (faster in single steps), FP64 and other edges run single steps. Results are
- * bit-identical either way; an odd step count ends with a single step. Resets the column shape to the
- * first in preference order that divides N (jp_autotune may pick another). */
+ * the 16-bit forms fuse only when a shape is picked (jp_set_step_pair_shape) or jp_autotune keeps one;
+ * FP64 and other edges run single steps. Results are bit-identical either way; an odd step count ends
+ * with a single step. Resets the column shape to the first in preference order that divides N. */
 int jp_set_step_pairs(jp_ctx* c, int on);
 /* Selects the fused pair's column shape by its index in the library's preference order (0: 128 x 4,
  * 1: 64 x 8, 2: 32 x 16 cells, each with the longest z-segment of 256, 128, 64 or 32 planes that divides
  * N; 3-5: the same columns with half that segment, at least 32 planes; -1: the first that applies) and
  * turns the pairs on (not in the
  * reference; a tuning knob, as jp_set_variant). JP_ERR_ARG when the shape does not divide N or the
- * storage form does not fuse; JP_ERR_STATE on an AMR context. Every shape computes the same result. */
+ * storage form does not fuse (FP64); JP_ERR_STATE on an AMR context. Every shape computes the same
+ * result, in every storage form. */
 int jp_set_step_pair_shape(jp_ctx* c, int pick);
 /* The fused pair's column shape {x, y, z extent, lanes per workgroup}; all zero when steps run singly. */
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]);

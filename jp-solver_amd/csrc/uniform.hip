@@ -424,9 +424,13 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 
 // Column shapes of the fused pair, in order of preference (measured on MI355X at 512^3 FP32: 128 x 4
 // with 832 lanes 58.3 GLUPS, 64 x 8 / 704 lanes 54.3, against 34.2 for single steps; taller columns need
-// two phase-A passes per lane, whose doubled prefetch registers spill at 1024 lanes). Only FP32 fuses: the 16-bit forms move half the bytes, so their fused pass (2.5 collisions per
-// cell where the single step has 1) is bound by arithmetic and ran slower than single steps in the same
-// process (FP16 64 x 16 / 1024 lanes 68.6 against 70.9 GLUPS); FP64 needs twice the LDS per cell.
+// two phase-A passes per lane, whose doubled prefetch registers spill at 1024 lanes). FP32 fuses by default.
+// The 16-bit forms move half the bytes, so their fused pass (2.5 collisions per cell where the single step
+// has 1) is bound by arithmetic (FP16 128 x 4: 76 % of the VALU issue slots, against 52 % for FP32): they
+// fuse only when a shape is picked (jp_set_step_pair_shape) or the autotuner keeps one. FP64 would need
+// twice the LDS per cell. Tried and dropped: buffer addressing (one descriptor per z-layer, 32-bit lane
+// offsets) cut the pair's registers from 102 to 69 and allowed two workgroups per CU, but ran no faster
+// (FP32 128 x 4: 63.3 against 64.4 GLUPS with global addresses).
 struct PairShape {
   int tx, ty, bs;
 };
@@ -436,7 +440,8 @@ constexpr int kShapeCount = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 // same with half that segment (at least 32 planes), for the autotuner to try where boxes differ.
 int step_pair_shape_count() { return 2 * kShapeCount; }
 bool step_pair_shape(int prec, int N, int pick, int* out4) {
-  if (prec != 0) return false;
+  if (prec == 3) return false;
+  if (prec != 0 && pick < 0) return false;  // the 16-bit forms fuse on request only (a pick, or jp_autotune)
   // z-segment: the longest of 256, 128, 64, 32 planes that divides N (at 512^3 on padded z-layers 256 planes
   // ran 62.0 against 61.2 GLUPS for 128: two halo planes per segment are 0.8 % of the reads instead of 1.6 %)
   const int tz0 = N % 256 == 0 ? 256 : (N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0)));
@@ -463,13 +468,23 @@ static hipError_t pair_launch(const void* fa, void* fb, const Dense& g, int tz, 
                      static_cast<raw*>(fb), g, uint32_t(tz), typename S::real(omega), enc, dec);
   return hipGetLastError();
 }
+template <class S>
+static hipError_t pair_shape(const void* fa, void* fb, const Dense& g, const int* s, double omega, float enc, float dec,
+                             hipStream_t st) {
+  if (s[0] == 128) return pair_launch<S, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
+  if (s[0] == 64) return pair_launch<S, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
+  return pair_launch<S, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
+}
 hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
                            int pick, hipStream_t st) {
   int s[4];
   if (!step_pair_shape(prec, int(g.N), pick, s)) return hipErrorInvalidValue;
-  if (s[0] == 128) return pair_launch<FormF32, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
-  if (s[0] == 64) return pair_launch<FormF32, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
-  return pair_launch<FormF32, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
+  switch (prec) {
+    case 0: return pair_shape<FormF32>(fa, fb, g, s, omega, enc, dec, st);
+    case 1: return pair_shape<FormF16>(fa, fb, g, s, omega, enc, dec, st);
+    case 2: return pair_shape<FormBF16>(fa, fb, g, s, omega, enc, dec, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <class S, int BS, int V, bool NL, bool NS>

@@ -125,16 +125,35 @@ def test_pair_shape_setter_selects_each_shape():
     a.set_step_pair_shape(-1)
     assert a.step_pair_shape() == (128, 4, 256, 832)
     a.close()
-    with pytest.raises(jpabi.JPError):
-        jpabi.Uniform(32).set_step_pair_shape(3)  # a 16-plane segment is below the minimum
-    with pytest.raises(jpabi.JPError) as e:
-        jpabi.Uniform(96).set_step_pair_shape(0)  # 128 does not divide 96
-    assert e.value.code == jpabi.JP_ERR_ARG
-    with pytest.raises(jpabi.JPError):
-        jpabi.Uniform(64, prec=jpabi.PREC_FP16).set_step_pair_shape(0)
-    with pytest.raises(jpabi.JPError) as e:
-        jpabi.Amr(16, l1=jpabi.MODE_SLAB, maxgb=4.0).set_step_pair_shape(0)
-    assert e.value.code == jpabi.JP_ERR_STATE
+    refused = [(jpabi.Uniform(32), 5, jpabi.JP_ERR_ARG),  # 32 x 16 divides 32, but its 16-plane half segment is
+                                                          # below the 32-plane minimum
+               (jpabi.Uniform(96), 0, jpabi.JP_ERR_ARG),  # 128 does not divide 96
+               (jpabi.Uniform(64, prec=jpabi.PREC_FP64), 0, jpabi.JP_ERR_ARG),  # FP64 does not fuse
+               (jpabi.Amr(16, l1=jpabi.MODE_SLAB, maxgb=4.0), 0, jpabi.JP_ERR_STATE)]
+    for ctx, pick, code in refused:
+        with pytest.raises(jpabi.JPError) as e:
+            ctx.set_step_pair_shape(pick)
+        assert e.value.code == code
+        ctx.close()
+
+
+@pytest.mark.parametrize("prec", [jpabi.PREC_FP16, jpabi.PREC_BF16])
+@pytest.mark.parametrize("pick", [0, 1, 2, 4])
+def test_sixteen_bit_fused_pairs_are_bit_exact(prec, pick):
+    """The 16-bit forms run single steps by default and fuse pairs when a shape is picked: the intermediate
+    step passes through the storage form in LDS, so the fused pass is bit-exact with the oracle."""
+    n = 128
+    o = orc.Uniform(n, re=1600, prec=prec)
+    o.init()
+    g = jpabi.Uniform(n, re=1600, prec=prec)
+    assert g.step_pair_shape() == (0, 0, 0, 0)
+    g.set_step_pair_shape(pick)
+    assert g.step_pair_shape()[0] > 0
+    g.set_raw(o.raw())
+    o.step(5)
+    g.step(5)
+    assert np.array_equal(g.raw(), o.raw())
+    g.close()
 
 
 @pytest.mark.parametrize("n,prec", [(256, jpabi.PREC_FP32), (96, jpabi.PREC_FP32), (64, jpabi.PREC_FP16)])
@@ -149,7 +168,7 @@ def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
     assert np.array_equal(g.raw(), o.raw())
     assert g.step_pair_shape() == shape
     valid = {(0, 0, 0, 0)}
-    if prec == jpabi.PREC_FP32:
+    if prec in (jpabi.PREC_FP32, jpabi.PREC_FP16, jpabi.PREC_BF16):
         tz = next(t for t in (256, 128, 64, 32) if n % t == 0)  # the longest z-segment that divides n
         valid |= {(x, y, t, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
                   for t in (tz, tz // 2) if n % x == 0 and n % y == 0 and t >= 32}
