@@ -218,16 +218,26 @@ struct FormBF16 {
   template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
 };
 
+// Fused multiply-add in the working type (one rounding, IEEE fusedMultiplyAdd).
+template <class R> __device__ __forceinline__ R fmx(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
 // BGK collision of one cell in the form's working representation (src/tgv3d_uniform.cu:55-60,
-// src/tgv3d_amr.cu:268-273): post<q>(x) = x - omega (x - eq_q). The shifted forms collide the
-// deviations d_q = f_q - w_q: rho - 1 = sum d_q, rho u = sum c_q d_q, eq_q = feq_q - w_q. In exact
-// arithmetic this is the same collision; in FP32 every operand is then of the order of the deviations,
-// whereas summing w_q + d_q rounds rho at the fixed low bits of the w_q, a bias of ~1e-7 per cell and
-// step that would show as a linear drift of the mass.
+// src/tgv3d_amr.cu:268-273): post<q>(x) = x - omega (x - feq_q), feq_q = w_q rho (1 + cu + cu^2/2 - 1.5 u^2),
+// cu = 3 c_q.u, with the multiply-adds fused at fixed points (the reference's compiler contracts them too,
+// nvcc's default --fmad=true; the oracle writes the same fused operations, so the two agree bit for bit):
+//   u.u = fma(uz, uz, fma(uy, uy, ux ux)), t = 1.5 u.u, s_q = fma(cu/2, cu, 1 - t),
+//   post<q>(x) = fma(-omega, fma(-w_q rho, s_q + cu, x), x).
+// The shifted forms collide the deviations d_q = f_q - w_q: rho - 1 = sum d_q, rho u = sum c_q d_q, and
+// feq_q - w_q = w_q ((rho - 1) + rho (cu + cu^2/2 - 1.5 u^2)) enters as fma(-w_q, fma(rho, g_q + cu,
+// rho - 1), d_q) with g_q = fma(cu/2, cu, -t). In exact arithmetic this is the same collision; in FP32
+// every operand is then of the order of the deviations, whereas summing w_q + d_q rounds rho at the fixed
+// low bits of the w_q, a bias of ~1e-7 per cell and step that would show as a linear drift of the mass.
 template <class S> struct Collider {
   using R = typename S::real;
-  EqPre<R> e;
-  R drho, rho;
+  R ux, uy, uz, t15, omt, wr[3], drho, rho;
   __device__ explicit Collider(const R* fin) {
     const Mom<R> m = moments(fin);
     if constexpr (S::kShift) {
@@ -237,32 +247,42 @@ template <class S> struct Collider {
       drho = R(0);
       rho = m.rho;
     }
-    e = eq_pre<R>(rho, m.ux / rho, m.uy / rho, m.uz / rho);
+    ux = m.ux / rho;
+    uy = m.uy / rho;
+    uz = m.uz / rho;
+    t15 = R(1.5) * fmx<R>(uz, uz, fmx<R>(uy, uy, ux * ux));
+    omt = R(1) - t15;
+    wr[0] = W<R>(0) * rho;
+    wr[1] = W<R>(1) * rho;
+    wr[2] = W<R>(7) * rho;
+  }
+  // the relaxed value of direction q from sc = s_q + cu (plain forms) or g_q + cu (shifted forms)
+  template <int q> __device__ R relax(R x, R om, R sc) const {
+    constexpr int cls = q == 0 ? 0 : (q < 7 ? 1 : 2);
+    if constexpr (S::kShift) return fmx<R>(-om, fmx<R>(-W<R>(q), fmx<R>(rho, sc, drho), x), x);
+    else return fmx<R>(-om, fmx<R>(-wr[cls], sc, x), x);
   }
   template <int q> __device__ R post(R x, R om) const {
-    if constexpr (S::kShift) return x - om * (x - deq<q>(e, drho, rho));
-    else return x - om * (x - feq<q>(e));
+    if constexpr (q == 0) {
+      return relax<0>(x, om, S::kShift ? -t15 : omt);
+    } else {
+      const R cu = R(3) * cdot<q>(ux, uy, uz);
+      const R hc = R(0.5) * cu;
+      return relax<q>(x, om, fmx<R>(hc, cu, S::kShift ? -t15 : omt) + cu);
+    }
   }
-  // post<q> of all 19 directions in place. The opposite directions q and q + 1 (q odd) share 3 c.u and
-  // 0.5 (3 c.u)^2: c_{q+1} = -c_q makes every term of c.u change sign exactly, and (-a) b = -(a b) in
-  // IEEE arithmetic, so each value equals post<q>'s bit for bit, with three fewer operations per pair.
+  // post<q> of all 19 directions in place. The opposite directions q and q + 1 (q odd) share cu/2 and
+  // s_q (g_q): c_{q+1} = -c_q makes cu change sign exactly, (-a)(-b) = ab in IEEE arithmetic, and
+  // s_q - cu = s_q + (-cu), so each value equals post<q>'s bit for bit, with two fewer operations per pair.
   __device__ void post_all(R* f, R om) const {
     f[0] = post<0>(f[0], om);
     sfor<9>([&](auto pc) {
       constexpr int q = 1 + 2 * decltype(pc)::value;
-      constexpr int cls = q < 7 ? 1 : 2;
-      const R cu = R(3) * cdot<q>(e.ux, e.uy, e.uz);
-      const R h = R(0.5) * cu * cu;
-      R ep, em;
-      if constexpr (S::kShift) {
-        ep = W<R>(q) * (drho + rho * ((cu + h) - e.t15));
-        em = W<R>(q) * (drho + rho * ((-cu + h) - e.t15));
-      } else {
-        ep = e.wr[cls] * (((R(1) + cu) + h) - e.t15);
-        em = e.wr[cls] * (((R(1) - cu) + h) - e.t15);
-      }
-      f[q] = f[q] - om * (f[q] - ep);
-      f[q + 1] = f[q + 1] - om * (f[q + 1] - em);
+      const R cu = R(3) * cdot<q>(ux, uy, uz);
+      const R hc = R(0.5) * cu;
+      const R s = fmx<R>(hc, cu, S::kShift ? -t15 : omt);
+      f[q] = relax<q>(f[q], om, s + cu);
+      f[q + 1] = relax<q + 1>(f[q + 1], om, s - cu);
     });
   }
 };

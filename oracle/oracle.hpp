@@ -188,26 +188,44 @@ template <class R> inline R deq_q(int q, R drho, R rho, R ux, R uy, R uz) {
   return wq<R>(q) * (drho + rho * (cu + R(0.5) * cu * cu - R(1.5) * (ux * ux + uy * uy + uz * uz)));
 }
 // BGK collision of one cell in the form's working representation, in place
-// (src/tgv3d_uniform.cu:55-60; src/tgv3d_amr.cu:268-273). The shifted forms collide the deviations
-// d_q = f_q - w_q (rho - 1 = sum d_q, rho u = sum c_q d_q): the same collision in exact arithmetic, but
-// in FP32 summing w_q + d_q would round rho at the fixed low bits of the w_q, a per-step bias of ~1e-7
-// that drifts the mass linearly.
+// (src/tgv3d_uniform.cu:55-60; src/tgv3d_amr.cu:268-273): f_q - omega (f_q - feq_q), feq_q =
+// w_q rho (1 + cu + cu^2/2 - 1.5 u^2), cu = 3 c_q.u, evaluated with fused multiply-adds at fixed points:
+//   u.u = fma(uz, uz, fma(uy, uy, ux ux)), t = 1.5 u.u, s_q = fma(cu/2, cu, 1 - t),
+//   d_q = fma(-w_q rho, s_q + cu, f_q) (= f_q - feq_q), f_q' = fma(-omega, d_q, f_q).
+// The reference's compiler contracts these multiply-adds too (nvcc's default --fmad=true); writing them
+// out fixes the rounding so that the GPU and this restatement agree bit for bit, and halves the
+// collision's instruction count. std::fma and the GPU's fma both round once (IEEE fusedMultiplyAdd).
+// The shifted forms collide the deviations d_q = f_q - w_q (rho - 1 = sum d_q, rho u = sum c_q d_q), with
+// feq_q - w_q = w_q ((rho - 1) + rho (cu + cu^2/2 - 1.5 u^2)) as fma(-w_q, fma(rho, g_q + cu, rho - 1),
+// d_q), g_q = fma(cu/2, cu, -t): the same collision in exact arithmetic, but in FP32 summing w_q + d_q
+// would round rho at the fixed low bits of the w_q, a per-step bias of ~1e-7 that drifts the mass.
 template <class S> inline void collide(typename S::real* fin, typename S::real om) {
   using R = typename S::real;
-  R rho, ux, uy, uz;
+  R rho, ux, uy, uz, drho = R(0);
   moments(fin, rho, ux, uy, uz);
   if (S::shift) {
-    const R drho = rho;
+    drho = rho;
     rho = R(1) + drho;
-    ux /= rho;
-    uy /= rho;
-    uz /= rho;
-    for (int q = 0; q < Q; ++q) fin[q] = fin[q] - om * (fin[q] - deq_q<R>(q, drho, rho, ux, uy, uz));
-  } else {
-    ux /= rho;
-    uy /= rho;
-    uz /= rho;
-    for (int q = 0; q < Q; ++q) fin[q] = fin[q] - om * (fin[q] - feq_q<R>(q, rho, ux, uy, uz));
+  }
+  ux /= rho;
+  uy /= rho;
+  uz /= rho;
+  const R t15 = R(1.5) * std::fma(uz, uz, std::fma(uy, uy, ux * ux));
+  const R omt = R(1) - t15;
+  for (int q = 0; q < Q; ++q) {
+    const R f = fin[q];
+    R d;
+    if (q == 0) {
+      d = S::shift ? std::fma(-wq<R>(0), std::fma(rho, -t15, drho), f) : std::fma(-(wq<R>(0) * rho), omt, f);
+    } else {
+      const R cu = R(3) * (R(cvx[q]) * ux + R(cvy[q]) * uy + R(cvz[q]) * uz);
+      const R hc = R(0.5) * cu;
+      if (S::shift)
+        d = std::fma(-wq<R>(q), std::fma(rho, std::fma(hc, cu, -t15) + cu, drho), f);
+      else
+        d = std::fma(-(wq<R>(q) * rho), std::fma(hc, cu, omt) + cu, f);
+    }
+    fin[q] = std::fma(-om, d, f);
   }
 }
 inline int pmod(int a, int n) { return ((a % n) + n) % n; }
