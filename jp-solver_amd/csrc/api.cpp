@@ -333,8 +333,8 @@ static int eager_steps(jp_ctx* c, long n) {
   return JP_OK;
 }
 
-// The buffers a pair of steps starts from: after two steps every ping-pong pair is back in its role, so
-// a captured pair applies exactly when the current roles equal those it was captured with.
+// The buffers a run of steps starts from: after an even number of steps every fine ping-pong pair is back
+// in its role, so a captured run applies exactly when the current roles equal those it was captured with.
 static void graph_key(const jp_ctx* c, const void* key[3]) {
   if (c->kind == 2) {
     ens_graph_key(c, key);
@@ -345,24 +345,26 @@ static void graph_key(const jp_ctx* c, const void* key[3]) {
   key[2] = c->kind == 1 ? c->amr.f2a : nullptr;
 }
 
-// The captured pair of steps for the current roles, captured on first use. A few pairs are kept: the
-// two parities of the dense buffers and, with device-resident regrids (which swap both fine pairs), of
-// the fine buffers. Capturing leaves the dense roles as the captured steps leave them; returning a kept
-// pair sets them the same way, before its replay is enqueued.
-static int graph_for(jp_ctx* c, hipGraphExec_t* out) {
+// The captured run of `len` steps (even) for the current roles, captured on first use. A few runs are
+// kept: per length, the two parities of the dense buffers and, with device-resident regrids (which swap
+// both fine pairs), of the fine buffers. Capturing leaves the dense roles as the captured steps leave
+// them; returning a kept run sets them the same way, before its replay is enqueued.
+constexpr long kGraphSteps = 16;  // longest captured run
+
+static int graph_for(jp_ctx* c, long len, hipGraphExec_t* out) {
   const void* key[3];
   graph_key(c, key);
   for (const auto& g : c->graphs)
-    if (g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2]) {
+    if (g.len == len && g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2]) {
       *out = g.exec;
       c->c0a = g.c0[0];
       c->c0b = g.c0[1];
       return JP_OK;
     }
-  if (c->graphs.size() >= 8) drop_graph(c);
+  if (c->graphs.size() >= 12) drop_graph(c);
   hipGraph_t g;
   HCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  const int rc = eager_steps(c, 2);
+  const int rc = eager_steps(c, len);
   hipError_t ec = hipStreamEndCapture(c->st, &g);
   if (rc) return rc;
   if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
@@ -370,7 +372,7 @@ static int graph_for(jp_ctx* c, hipGraphExec_t* out) {
   ec = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ec != hipSuccess) return fail(c, JP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ec));
-  c->graphs.push_back({{key[0], key[1], key[2]}, exec, {c->c0a, c->c0b}});
+  c->graphs.push_back({{key[0], key[1], key[2]}, len, exec, {c->c0a, c->c0b}});
   HCK(c, hipGraphUpload(exec, c->st));
   *out = exec;
   return JP_OK;
@@ -382,12 +384,17 @@ int jp_step(jp_ctx* c, long n) {
   if (c->kind == 2)
     if (int r = ens_check(c)) return r;
   if (!c->use_graph) return eager_steps(c, n);
-  for (long p = 0; p < n / 2; ++p) {
+  // replays of captured runs of up to kGraphSteps steps (the gap between two replays costs more than
+  // the launches inside one), then a single eager step for an odd rest
+  long rem = n;
+  while (rem >= 2) {
+    const long len = std::min(rem & ~1L, kGraphSteps);
     hipGraphExec_t g;
-    if (int r = graph_for(c, &g)) return r;
+    if (int r = graph_for(c, len, &g)) return r;
     HCK(c, hipGraphLaunch(g, c->st));
+    rem -= len;
   }
-  return n % 2 ? eager_steps(c, 1) : JP_OK;
+  return rem ? eager_steps(c, 1) : JP_OK;
 }
 
 int jp_set_variant(jp_ctx* c, int variant) {

@@ -68,12 +68,13 @@ struct jp_ctx {
   size_t host_out_bytes = 0;
   double* partial = nullptr;  // diagnostic partial sums (3 doubles per slot), then the 3 results
   size_t partial_slots = 0;
-  // Captured pairs of steps, keyed by the buffers they start from (the ping-pong roles), so that an odd
-  // step or a device-resident regrid selects another pair instead of recapturing. c0 holds the dense
-  // pair's roles after the captured steps (two single steps restore them, a fused pair swaps them once),
-  // which a replay applies.
+  // Captured runs of an even number of steps, keyed by their length and the buffers they start from (the
+  // ping-pong roles), so that an odd step or a device-resident regrid selects another run instead of
+  // recapturing. c0 holds the dense pair's roles after the captured steps (two single steps restore them,
+  // a fused pair swaps them once), which a replay applies.
   struct Graph {
     const void* key[3];
+    long len;  // steps captured (even)
     hipGraphExec_t exec;
     void* c0[2];
   };
