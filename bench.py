@@ -276,6 +276,7 @@ def main():
     g.init()
     # the library's autotuner times single steps and the fused column shapes that divide N, keeps the fastest
     shape = g.autotune()  # (x, y, z, lanes) of the fused column; zeros for single steps
+    tuned = g.tuned
     print(f"# uniform {n}^3 {args.prec}: launch form {shape}, warmup {args.warmup}, timing {args.steps} steps",
           file=sys.stderr, flush=True)
     per_launch = 2 if shape[0] else 1
@@ -316,7 +317,8 @@ def main():
                    # the library's default device layout for the form (jp_uniform_params.plane_pad = 0)
                    "layout": "direction planes" if args.prec == "fp64" else "z-layers",
                    "launch": (f"two fused steps per launch, columns {shape[0]} x {shape[1]} x {shape[2]}, "
-                              f"{shape[3]} lanes" if shape[0] else "one step per launch") + " (picked by jp_autotune)"},
+                              f"{shape[3]} lanes" if shape[0] else f"one step per launch, variant {tuned['variant']}")
+                             + f", {tuned['waves_per_cu']} waves per CU (picked by jp_autotune)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic},
         "extra": {"copy_kernel_gbps": copy_gbps, "attainable_hbm_gbps": attainable,

@@ -182,9 +182,13 @@ int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
  *   fill, level-2 fill, level-2 step, level-2 step + restriction, level-1 step, level-1 step +
  *   restriction). The two launches with a restriction write the parent level, so they are not timed:
  *   they always run one workgroup per tile, which overlaps each tile's restriction tail.
- * - Uniform lattice: picks how steps run, as single steps or as fused pairs with one of the column shapes
- *   that divide the edge, and keeps the fastest. groups[0..3] receive the chosen column shape (x, y,
- *   z extent, lanes; zeros for single steps) and groups[4..6] zero.
+ * - Uniform lattice: picks how steps run: single steps in one of 18 launch variants (jp_set_variant: 256,
+ *   512 or 1024 lanes per workgroup, 1, 2 or 8 cells per lane, streaming or cached stores), or fused
+ *   pairs with one of the column shapes that divide the edge; keeps the fastest, and the fastest single
+ *   variant for the odd step after pairs. groups[0..3] receive the chosen column shape (x, y, z extent,
+ *   lanes; zeros for single steps), groups[4] the single-step variant, groups[5] the waves per CU the kept
+ *   kernel runs at (workgroups a CU holds at once, from its registers, LDS and size, x waves per
+ *   workgroup), groups[6] zero.
  * An AMR context has no candidates (JP_ERR_STATE). */
 #define JP_ENS_KERNELS 7
 int jp_autotune(jp_ctx* c, int* groups);

@@ -247,27 +247,41 @@ int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member) {
 // Uniform lattice: times two steps in every form the lattice can take (single steps, and the fused pair
 // with each column shape that divides the edge), each launch re-run from the current state into the other
 // buffer so that the state is unchanged, and keeps the fastest (median of 3 after one untimed launch).
+// Single-step launch variants the uniform autotuner times (jp_set_variant bits): cached loads; 256, 512 or
+// 1024 lanes per workgroup; 1, 2 or 8 consecutive x-cells per lane; streaming or cached stores. Lanes per
+// workgroup and registers per lane fix the waves a CU holds.
+static const int kTuneVariants[] = {4, 6, 7, 12, 14, 15, 20, 22, 23, 28, 30, 31, 52, 54, 55, 60, 62, 63};
+
 static int uniform_autotune(jp_ctx* c, int* out) {
   const float enc = float(c->d.scale[0]), dec = float(1.0 / c->d.scale[0]);
   hipEvent_t a, b;
   HCK(c, hipEventCreate(&a));
   HCK(c, hipEventCreate(&b));
-  auto launch = [&](int cand) -> hipError_t {  // two steps of candidate cand (-1: single steps)
-    if (cand >= 0) return dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, cand, c->st);
-    for (int k = 0; k < 2; ++k)
-      if (hipError_t e = dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, c->variant, c->st)) return e;
+  struct Cand {
+    int pick, variant;  // pick >= 0: fused pair of column form pick; else single steps of launch variant
+  };
+  std::vector<Cand> cands;
+  for (int v : kTuneVariants) cands.push_back({-1, v});
+  for (int p = 0; p < step_pair_shape_count(); ++p) {
+    int s[4];
+    if (step_pair_shape(c->prec, c->d.n, p, s)) cands.push_back({p, c->variant});
+  }
+  // two steps of a candidate from the current state into the other buffer: the state is unchanged
+  auto launch = [&](const Cand& k) -> hipError_t {
+    if (k.pick >= 0) return dense_step_pair(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, k.pick, c->st);
+    for (int r = 0; r < 2; ++r)
+      if (hipError_t e = dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc, dec, k.variant, c->st)) return e;
     return hipSuccess;
   };
-  float best = 1e30f;
-  int pick = -1, shape[4] = {0, 0, 0, 0};
-  for (int cand = -1; cand < step_pair_shape_count(); ++cand) {
-    int s[4] = {0, 0, 0, 0};
-    if (cand >= 0 && !step_pair_shape(c->prec, c->d.n, cand, s)) continue;
-    HCK(c, launch(cand));
+  float best = 1e30f, best_single = 1e30f;
+  Cand keep{-1, c->variant};
+  int single_variant = c->variant;
+  for (const Cand& k : cands) {
+    HCK(c, launch(k));
     float t[3];
     for (int r = 0; r < 3; ++r) {
       HCK(c, hipEventRecord(a, c->st));
-      HCK(c, launch(cand));
+      HCK(c, launch(k));
       HCK(c, hipEventRecord(b, c->st));
       HCK(c, hipEventSynchronize(b));
       HCK(c, hipEventElapsedTime(&t[r], a, b));
@@ -275,17 +289,27 @@ static int uniform_autotune(jp_ctx* c, int* out) {
     std::sort(t, t + 3);
     if (t[1] < best) {
       best = t[1];
-      pick = cand;
-      for (int i = 0; i < 4; ++i) shape[i] = s[i];
+      keep = k;
+    }
+    if (k.pick < 0 && t[1] < best_single) {
+      best_single = t[1];
+      single_variant = k.variant;
     }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  c->step_pairs = pick >= 0;
-  c->pair_pick = pick;
+  c->step_pairs = keep.pick >= 0;
+  c->pair_pick = keep.pick;
+  c->variant = single_variant;  // also runs the odd step after the pairs
   drop_graph(c);
-  if (out)
-    for (int i = 0; i < JP_ENS_KERNELS; ++i) out[i] = i < 4 ? shape[i] : 0;
+  if (out) {
+    int shape[4] = {0, 0, 0, 0};
+    if (keep.pick >= 0) step_pair_shape(c->prec, c->d.n, keep.pick, shape);
+    for (int i = 0; i < 4; ++i) out[i] = shape[i];
+    out[4] = c->variant;
+    out[5] = dense_step_waves_per_cu(c->prec, c->g0, c->variant, keep.pick);
+    out[6] = 0;
+  }
   return JP_OK;
 }
 

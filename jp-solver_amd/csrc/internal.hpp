@@ -68,6 +68,9 @@ hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double
 // FP64, an edge no shape divides, or a pick that does not divide it): the lattice runs single steps.
 int step_pair_shape_count();
 bool step_pair_shape(int prec, int N, int pick, int* out4);
+// Waves per CU the single step (pick < 0, launch variant `variant`) or the fused pair (column form `pick`)
+// runs at: workgroups a CU holds at once (registers, LDS, workgroup size) x waves per workgroup; 0 if none.
+int dense_step_waves_per_cu(int prec, const Dense& g, int variant, int pick);
 // Two steps fa -> (t + 1) -> fb in one launch, bit-identical to two dense_step calls; needs a shape.
 hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
                            int pick, hipStream_t st);

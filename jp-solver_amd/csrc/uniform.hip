@@ -459,10 +459,12 @@ bool step_pair_shape(int prec, int N, int pick, int* out4) {
   return false;
 }
 
+// occ != nullptr: instead of launching, *occ = workgroups of this kernel one CU holds at once
 template <class S, int TX, int TY, int BS>
 static hipError_t pair_launch(const void* fa, void* fb, const Dense& g, int tz, double omega, float enc, float dec,
-                              hipStream_t st) {
+                              hipStream_t st, int* occ = nullptr) {
   using raw = typename S::raw;
+  if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step2<S, TX, TY, BS>, BS, 0);
   const uint32_t nt = (g.N / TX) * (g.N / TY) * (g.N / uint32_t(tz));
   hipLaunchKernelGGL((k_dense_step2<S, TX, TY, BS>), dim3(nt), dim3(BS), 0, st, static_cast<const raw*>(fa),
                      static_cast<raw*>(fb), g, uint32_t(tz), typename S::real(omega), enc, dec);
@@ -470,27 +472,36 @@ static hipError_t pair_launch(const void* fa, void* fb, const Dense& g, int tz, 
 }
 template <class S>
 static hipError_t pair_shape(const void* fa, void* fb, const Dense& g, const int* s, double omega, float enc, float dec,
-                             hipStream_t st) {
-  if (s[0] == 128) return pair_launch<S, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st);
-  if (s[0] == 64) return pair_launch<S, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st);
-  return pair_launch<S, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st);
+                             hipStream_t st, int* occ) {
+  if (s[0] == 128) return pair_launch<S, 128, 4, 832>(fa, fb, g, s[2], omega, enc, dec, st, occ);
+  if (s[0] == 64) return pair_launch<S, 64, 8, 704>(fa, fb, g, s[2], omega, enc, dec, st, occ);
+  return pair_launch<S, 32, 16, 640>(fa, fb, g, s[2], omega, enc, dec, st, occ);
 }
-hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
-                           int pick, hipStream_t st) {
+static hipError_t pair_dispatch(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
+                                int pick, hipStream_t st, int* occ) {
   int s[4];
   if (!step_pair_shape(prec, int(g.N), pick, s)) return hipErrorInvalidValue;
   switch (prec) {
-    case 0: return pair_shape<FormF32>(fa, fb, g, s, omega, enc, dec, st);
-    case 1: return pair_shape<FormF16>(fa, fb, g, s, omega, enc, dec, st);
-    case 2: return pair_shape<FormBF16>(fa, fb, g, s, omega, enc, dec, st);
+    case 0: return pair_shape<FormF32>(fa, fb, g, s, omega, enc, dec, st, occ);
+    case 1: return pair_shape<FormF16>(fa, fb, g, s, omega, enc, dec, st, occ);
+    case 2: return pair_shape<FormBF16>(fa, fb, g, s, omega, enc, dec, st, occ);
   }
   return hipErrorInvalidValue;
 }
+hipError_t dense_step_pair(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec,
+                           int pick, hipStream_t st) {
+  return pair_dispatch(prec, fa, fb, g, omega, enc, dec, pick, st, nullptr);
+}
 
 template <class S, int BS, int V, bool NL, bool NS>
-static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, hipStream_t st) {
+static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, hipStream_t st,
+                         int* occ) {
   using R = typename S::real;
   const uint64_t cells = uint64_t(g.N) * g.N * g.N;
+  if (occ) {  // workgroups one CU holds at once, instead of a launch
+    if constexpr (V == 1) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step<S, BS, NL, NS>, BS, 0);
+    else return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step_vec<S, BS, V, NL, NS>, BS, 0);
+  }
   if constexpr (V == 1)
     hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
                        static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
@@ -501,13 +512,13 @@ static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega,
 }
 template <class S, int V>
 static hipError_t step_bs(const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int v,
-                          hipStream_t st) {
+                          hipStream_t st, int* occ) {
   const bool nl = !(v & 4), ns = !(v & 8);
-#define JP_STEP_V(BS_)                                                                 \
-  if (nl && ns) return step_v<S, BS_, V, true, true>(fa, fb, g, omega, enc, dec, st);   \
-  if (nl) return step_v<S, BS_, V, true, false>(fa, fb, g, omega, enc, dec, st);        \
-  if (ns) return step_v<S, BS_, V, false, true>(fa, fb, g, omega, enc, dec, st);        \
-  return step_v<S, BS_, V, false, false>(fa, fb, g, omega, enc, dec, st);
+#define JP_STEP_V(BS_)                                                                      \
+  if (nl && ns) return step_v<S, BS_, V, true, true>(fa, fb, g, omega, enc, dec, st, occ);   \
+  if (nl) return step_v<S, BS_, V, true, false>(fa, fb, g, omega, enc, dec, st, occ);        \
+  if (ns) return step_v<S, BS_, V, false, true>(fa, fb, g, omega, enc, dec, st, occ);        \
+  return step_v<S, BS_, V, false, false>(fa, fb, g, omega, enc, dec, st, occ);
   switch (v & 3) {
     case 1: { JP_STEP_V(128) }
     case 2: { JP_STEP_V(512) }
@@ -519,7 +530,7 @@ static hipError_t step_bs(const void* fa, void* fb, const Dense& g, double omega
 // variant: bits 0-1 workgroup size (256, 128, 512, 1024); bit 2 cached loads; bit 3 cached stores;
 // bits 4-5 cells per lane (1, 2, 4, 8). Shapes that do not tile N^3 fall back to smaller ones.
 template <class S> static hipError_t step_t(const void* fa, void* fb, const Dense& g, double omega, float enc,
-                                            float dec, int v, hipStream_t st) {
+                                            float dec, int v, hipStream_t st, int* occ, int* lanes) {
   const uint64_t cells = uint64_t(g.N) * g.N * g.N;
   int bs = (v & 3) == 1 ? 128 : (v & 3) == 2 ? 512 : (v & 3) == 3 ? 1024 : 256;
   int V = 1 << ((v >> 4) & 3);
@@ -529,22 +540,41 @@ template <class S> static hipError_t step_t(const void* fa, void* fb, const Dens
     v &= ~3;  // N^3 is a multiple of 512; 1024 needs an even tile count
     bs = 256;
   }
+  if (lanes) *lanes = bs;
   switch (V) {
-    case 2: return step_bs<S, 2>(fa, fb, g, omega, enc, dec, v, st);
-    case 4: return step_bs<S, 4>(fa, fb, g, omega, enc, dec, v, st);
-    case 8: return step_bs<S, 8>(fa, fb, g, omega, enc, dec, v, st);
-    default: return step_bs<S, 1>(fa, fb, g, omega, enc, dec, v, st);
+    case 2: return step_bs<S, 2>(fa, fb, g, omega, enc, dec, v, st, occ);
+    case 4: return step_bs<S, 4>(fa, fb, g, omega, enc, dec, v, st, occ);
+    case 8: return step_bs<S, 8>(fa, fb, g, omega, enc, dec, v, st, occ);
+    default: return step_bs<S, 1>(fa, fb, g, omega, enc, dec, v, st, occ);
   }
 }
 hipError_t dense_step(int prec, const void* fa, void* fb, const Dense& g, double omega, float enc, float dec, int variant,
                       hipStream_t st) {
   switch (prec) {
-    case 0: return step_t<FormF32>(fa, fb, g, omega, enc, dec, variant, st);
-    case 1: return step_t<FormF16>(fa, fb, g, omega, enc, dec, variant, st);
-    case 2: return step_t<FormBF16>(fa, fb, g, omega, enc, dec, variant, st);
-    case 3: return step_t<FormF64>(fa, fb, g, omega, enc, dec, variant, st);
+    case 0: return step_t<FormF32>(fa, fb, g, omega, enc, dec, variant, st, nullptr, nullptr);
+    case 1: return step_t<FormF16>(fa, fb, g, omega, enc, dec, variant, st, nullptr, nullptr);
+    case 2: return step_t<FormBF16>(fa, fb, g, omega, enc, dec, variant, st, nullptr, nullptr);
+    case 3: return step_t<FormF64>(fa, fb, g, omega, enc, dec, variant, st, nullptr, nullptr);
   }
   return hipErrorInvalidValue;
+}
+int dense_step_waves_per_cu(int prec, const Dense& g, int variant, int pick) {
+  int blocks = 0, lanes = 0, s[4];
+  hipError_t e = hipErrorInvalidValue;
+  if (pick >= 0) {
+    if (step_pair_shape(prec, int(g.N), pick, s)) {
+      e = pair_dispatch(prec, nullptr, nullptr, g, 0.0, 1.f, 1.f, pick, nullptr, &blocks);
+      lanes = s[3];
+    }
+  } else {
+    switch (prec) {
+      case 0: e = step_t<FormF32>(nullptr, nullptr, g, 0.0, 1.f, 1.f, variant, nullptr, &blocks, &lanes); break;
+      case 1: e = step_t<FormF16>(nullptr, nullptr, g, 0.0, 1.f, 1.f, variant, nullptr, &blocks, &lanes); break;
+      case 2: e = step_t<FormBF16>(nullptr, nullptr, g, 0.0, 1.f, 1.f, variant, nullptr, &blocks, &lanes); break;
+      case 3: e = step_t<FormF64>(nullptr, nullptr, g, 0.0, 1.f, 1.f, variant, nullptr, &blocks, &lanes); break;
+    }
+  }
+  return e == hipSuccess ? blocks * ((lanes + 63) / 64) : 0;
 }
 
 template <class S> static hipError_t macro_t(const void* f, const Dense& g, float dec, void* F, hipStream_t st) {

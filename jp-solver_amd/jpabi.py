@@ -288,11 +288,13 @@ class Uniform(_Ctx):
         self._check(rc)
 
     def autotune(self):
-        """Times single steps and every fused column shape that divides N on the current state (which it
-        leaves unchanged) and keeps the fastest; returns its shape (x, y, z extent, lanes), zeros for single
-        steps."""
+        """Times single steps in each launch variant and every fused column shape that divides N on the
+        current state (which it leaves unchanged) and keeps the fastest; returns its shape (x, y, z extent,
+        lanes), zeros for single steps. self.tuned then also holds the single-step variant and the waves per
+        CU of the kept kernel."""
         g = (ctypes.c_int * ENS_KERNELS)()
         self._check(lib().jp_autotune(self.h, g))
+        self.tuned = {"shape": tuple(g[:4]), "variant": g[4], "waves_per_cu": g[5]}
         return tuple(g[:4])
 
 

@@ -158,8 +158,9 @@ def test_sixteen_bit_fused_pairs_are_bit_exact(prec, pick):
 
 @pytest.mark.parametrize("n,prec", [(256, jpabi.PREC_FP32), (96, jpabi.PREC_FP32), (64, jpabi.PREC_FP16)])
 def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
-    """The uniform autotuner times single steps and the fused column shapes that divide N, leaves the state
-    unchanged, keeps one of them, and the lattice then steps bit-exactly as before."""
+    """The uniform autotuner times single steps in its launch variants and the fused column shapes that
+    divide N, leaves the state unchanged, keeps one of them (reporting its waves per CU), and the lattice
+    then steps bit-exactly as before."""
     o = orc.Uniform(n, re=1600, prec=prec)
     o.init()
     g = jpabi.Uniform(n, re=1600, prec=prec)
@@ -173,6 +174,10 @@ def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
         valid |= {(x, y, t, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
                   for t in (tz, tz // 2) if n % x == 0 and n % y == 0 and t >= 32}
     assert shape in valid
+    # the kept single-step variant is one of the timed ones, and the kernel holds at least one workgroup
+    assert g.tuned["variant"] in (4, 6, 7, 12, 14, 15, 20, 22, 23, 28, 30, 31, 52, 54, 55, 60, 62, 63)
+    lanes = shape[3] if shape[0] else (256, 128, 512, 1024)[g.tuned["variant"] & 3]
+    assert g.tuned["waves_per_cu"] >= lanes // 64 and g.tuned["waves_per_cu"] <= 32
     o.step(5)
     g.step(5)
     assert np.array_equal(g.raw(), o.raw())
