@@ -1,7 +1,7 @@
 """GPU parity of the uniform time step against the CPU oracle, through the C ABI.
 
 Bar: the FP32, FP16, BF16 and FP64 steps are bit-exact with the oracle from the same stored state (same
-operation order, no FMA contraction on either side); the TGV initial state agrees to within one unit in
+operation order, fused multiply-adds exactly where both write them out); the TGV initial state agrees to within one unit in
 the last place (device and host double-precision sin/cos may differ in the last bit); the full 64^3,
 Re 1600, t* = 2 run (407 steps) keeps populations within 1e-5 relative and the diagnostics within 1e-9.
 """
