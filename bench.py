@@ -279,7 +279,7 @@ def main():
     tuned = g.tuned
     print(f"# uniform {n}^3 {args.prec}: launch form {shape}, warmup {args.warmup}, timing {args.steps} steps",
           file=sys.stderr, flush=True)
-    per_launch = 2 if shape[0] else 1
+    per_launch = tuned["steps_per_launch"]
     r = g.bench(args.steps, warmup=args.warmup, graph=bool(args.graph))
     E, ens, mass = g.diag()
     g.close()

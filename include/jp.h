@@ -188,7 +188,7 @@ int jp_ensemble_member(jp_ctx* ens, int index, jp_ctx** member);
  *   variant for the odd step after pairs. groups[0..3] receive the chosen column shape (x, y, z extent,
  *   lanes; zeros for single steps), groups[4] the single-step variant, groups[5] the waves per CU the kept
  *   kernel runs at (workgroups a CU holds at once, from its registers, LDS and size, x waves per
- *   workgroup), groups[6] zero.
+ *   workgroup), groups[6] the steps per launch (1 or 2).
  * An AMR context has no candidates (JP_ERR_STATE). */
 #define JP_ENS_KERNELS 7
 int jp_autotune(jp_ctx* c, int* groups);
@@ -238,6 +238,9 @@ int jp_set_step_pairs(jp_ctx* c, int on);
 int jp_set_step_pair_shape(jp_ctx* c, int pick);
 /* The fused pair's column shape {x, y, z extent, lanes per workgroup}; all zero when steps run singly. */
 int jp_step_pair_shape(const jp_ctx* c, int shape[4]);
+/* Steps one launch of the uniform lattice advances: 1 (single steps) or 2 (fused pairs); 1 on AMR and
+ * ensemble contexts (not in the reference; reports the tuning knob above). */
+int jp_steps_per_launch(const jp_ctx* c, int* steps);
 
 /* Energy <|u|^2/2>, enstrophy <|omega|^2/2>, mass <rho>, reduced on the device in FP64 with a fixed
  * order; 24 bytes cross to the host (src/tgv3d_uniform.cu:170-183; src/tgv3d_amr.cu:882-885). */

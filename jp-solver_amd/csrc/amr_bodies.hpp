@@ -202,23 +202,62 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
                                       const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
   using R = typename S::real;
+  using raw = typename S::raw;
   const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
   R fin[Q];
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
-    if constexpr (q == 0) {
-      fin[0] = S::template in<0>(fa[tidx(b, 0, cell)], dec);
-    } else {
-      const int si = i - cx, sj = j - cy, sk = k - cz;
-      const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
-      const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
-      const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
-      const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-      if (t >= 0) fin[q] = S::template in<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
-      else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
-    }
-  });
+  if constexpr (sizeof(raw) == sizeof(R)) {
+    // populations and ghost values share a type: the compiler selects the address, one load per direction
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+      if constexpr (q == 0) {
+        fin[0] = S::template in<0>(fa[tidx(b, 0, cell)], dec);
+      } else {
+        const int si = i - cx, sj = j - cy, sk = k - cz;
+        const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
+        const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
+        const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
+        const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        if (t >= 0) fin[q] = S::template in<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
+        else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
+      }
+    });
+  } else {
+    // 16-bit populations, FP32 ghost values: every lane loads its population (a ghost lane its own cell's,
+    // discarded), only ghost lanes load the ghost record, and nothing is decoded before all loads are
+    // issued. Decoding inside the branch made each direction wait for its load: 19 round trips in series.
+    raw v[Q];
+    R gv[Q];
+    uint32_t from_ghost = 0;
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+      gv[q] = R(0);
+      if constexpr (q == 0) {
+        v[0] = fa[tidx(b, 0, cell)];
+      } else {
+        const int si = i - cx, sj = j - cy, sk = k - cz;
+        const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
+        const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
+        const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
+        const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        const bool same = t >= 0;
+        v[q] = fa[tidx(same ? t : b, q, same ? ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7) : cell)];
+        if (t < 0) {
+          gv[q] = g[ghost_slot(q, i, j, k)];
+          from_ghost |= 1u << q;
+        }
+      }
+    });
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      // the empty asm takes the ghost value as loaded: it keeps the compiler from moving the decode
+      // into the branch above, where it would wait for the load
+      R gq = gv[q];
+      asm volatile("" : "+v"(gq));
+      fin[q] = (from_ghost >> q) & 1u ? S::template from_full<q>(gq) : S::template in<q>(v[q], dec);
+    });
+  }
   collide_store<S, kKeep>(fin, om, fb, b, cell, enc, kept, dec);
 }
 

@@ -308,7 +308,7 @@ static int uniform_autotune(jp_ctx* c, int* out) {
     for (int i = 0; i < 4; ++i) out[i] = shape[i];
     out[4] = c->variant;
     out[5] = dense_step_waves_per_cu(c->prec, c->g0, c->variant, keep.pick);
-    out[6] = 0;
+    out[6] = keep.pick >= 0 ? 2 : 1;  // steps per launch
   }
   return JP_OK;
 }
@@ -453,6 +453,13 @@ int jp_step_pair_shape(const jp_ctx* c, int shape[4]) {
   if (!c || !shape) return JP_ERR_ARG;
   for (int i = 0; i < 4; ++i) shape[i] = 0;
   if (c->kind == 0 && c->step_pairs) step_pair_shape(c->prec, c->d.n, c->pair_pick, shape);
+  return JP_OK;
+}
+
+int jp_steps_per_launch(const jp_ctx* c, int* steps) {
+  if (!c || !steps) return JP_ERR_ARG;
+  int shape[4];
+  *steps = c->kind == 0 && c->step_pairs && step_pair_shape(c->prec, c->d.n, c->pair_pick, shape) ? 2 : 1;
   return JP_OK;
 }
 

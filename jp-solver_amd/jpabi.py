@@ -117,7 +117,7 @@ def lib():
             "jp_get_table": (i, [vp, i, vp, l, ctypes.POINTER(l)]),
             "jp_use_graph": (i, [vp, i]), "jp_set_variant": (i, [vp, i]),
             "jp_set_step_pairs": (i, [vp, i]), "jp_step_pair_shape": (i, [vp, vp]),
-            "jp_set_step_pair_shape": (i, [vp, i]),
+            "jp_set_step_pair_shape": (i, [vp, i]), "jp_steps_per_launch": (i, [vp, ctypes.POINTER(i)]),
             "jp_ensemble_dense_shape": (i, [vp, i, ctypes.POINTER(i)]), "jp_ensemble_create": (i, [vp, i, i, d, i, ctypes.POINTER(vp)]),
             "jp_ensemble_member": (i, [vp, i, ctypes.POINTER(vp)]), "jp_autotune": (i, [vp, vp]),
         }
@@ -269,6 +269,12 @@ class _Ctx:
         self._check(lib().jp_step_pair_shape(self.h, ctypes.cast(a, ctypes.c_void_p)))
         return tuple(a)
 
+    def steps_per_launch(self):
+        """1 for single steps, 2 for fused pairs."""
+        k = ctypes.c_int()
+        self._check(lib().jp_steps_per_launch(self.h, ctypes.byref(k)))
+        return k.value
+
     def sync(self):
         self._check(lib().jp_sync(self.h))
 
@@ -294,7 +300,7 @@ class Uniform(_Ctx):
         CU of the kept kernel."""
         g = (ctypes.c_int * ENS_KERNELS)()
         self._check(lib().jp_autotune(self.h, g))
-        self.tuned = {"shape": tuple(g[:4]), "variant": g[4], "waves_per_cu": g[5]}
+        self.tuned = {"shape": tuple(g[:4]), "variant": g[4], "waves_per_cu": g[5], "steps_per_launch": g[6]}
         return tuple(g[:4])
 
 

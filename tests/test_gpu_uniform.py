@@ -122,6 +122,9 @@ def test_pair_shape_setter_selects_each_shape():
         a.set_raw(start)
         a.step(5)  # two pairs and a single step
         assert np.array_equal(a.raw(), expected), shape
+    assert a.steps_per_launch() == 2
+    a.set_step_pairs(False)
+    assert a.steps_per_launch() == 1
     a.set_step_pair_shape(-1)
     assert a.step_pair_shape() == (128, 4, 256, 832)
     a.close()
@@ -174,6 +177,8 @@ def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
         valid |= {(x, y, t, lanes) for x, y, lanes in [(128, 4, 832), (64, 8, 704), (32, 16, 640)]
                   for t in (tz, tz // 2) if n % x == 0 and n % y == 0 and t >= 32}
     assert shape in valid
+    assert g.tuned["steps_per_launch"] == g.steps_per_launch()
+    assert g.tuned["steps_per_launch"] == (2 if shape[0] else 1)
     # the kept single-step variant is one of the timed ones, and the kernel holds at least one workgroup
     assert g.tuned["variant"] in (4, 6, 7, 12, 14, 15, 20, 22, 23, 28, 30, 31, 52, 54, 55, 60, 62, 63)
     lanes = shape[3] if shape[0] else (256, 128, 512, 1024)[g.tuned["variant"] & 3]
