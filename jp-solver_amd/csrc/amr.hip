@@ -119,7 +119,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
   });
 }
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kLoop && sizeof(typename S::raw) == 2 ? 7 : (sizeof(typename S::raw) == 8 ? 4 : 8)))) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                            const typename S::real* __restrict__ gh2,
                                                            const int* __restrict__ nb2, int n2, const int* __restrict__ dn2,
                                                            int w2, typename S::real om2, float enc2, float dec2,
