@@ -62,10 +62,11 @@ uint64_t default_layer_pad(int prec, int N) {
 }
 
 double default_scale(double u0) { return std::exp2(std::floor(std::log2(2048.0 / u0))); }
-// Dense-step launch variant per storage form, from the 512^3 sweep on MI355X: FP32 8 cells per lane in
-// 512-lane groups with cached loads and stores; 16-bit forms one cell per lane, cached loads, streaming
-// stores; FP64 one cell per lane, cached loads and stores.
-int default_variant(int prec) { return prec == 0 ? 62 : (prec == 3 ? 12 : 6); }
+// Dense-step launch variant per storage form, from the 512^3 sweeps on MI355X: one cell per lane with
+// cached loads; 512-lane groups and streaming stores for FP32 and the 16-bit forms (at 512^3 a group is one
+// x-row, so the loads take scalar row bases: FP32 37.2 GLUPS against 34.0 for 8 cells per lane, FP16 79.1
+// against 73.2 for 1024-lane groups), 256-lane groups and cached stores for FP64.
+int default_variant(int prec) { return prec == 3 ? 12 : 6; }
 
 int drop_graph(jp_ctx* c) {
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);

@@ -42,6 +42,44 @@ __global__ __launch_bounds__(BS) void k_dense_step(const typename S::raw* __rest
   dense_step_cell<S, kNtLoad, kNtStore>(fa, fb, g, omega, enc, dec, blockIdx.x * BS + threadIdx.x);
 }
 
+// The same when a workgroup lies inside one x-row (N a multiple of BS): y and z are the workgroup's, so
+// every direction's source row starts at a workgroup-uniform address (scalar registers) and a lane adds
+// only its 32-bit x offset, one of three per lane. The loads and stores then take the scalar-base address
+// form and need no 64-bit lane arithmetic (about 95 of the 16-bit step's 425 vector instructions per cell
+// in the form above). Each cell's arithmetic is dense_step_cell's.
+template <class S, int BS, bool kNtLoad, bool kNtStore>
+__global__ __launch_bounds__(BS) void k_dense_step_row(const typename S::raw* __restrict__ fa,
+                                                       typename S::raw* __restrict__ fb, Dense g,
+                                                       typename S::real omega, float enc, float dec) {
+  using R = typename S::real;
+  using raw = typename S::raw;
+  const uint32_t N = g.N, per_row = N / BS;
+  const uint32_t row = blockIdx.x / per_row, x = (blockIdx.x - row * per_row) * BS + threadIdx.x;
+  const uint32_t z = fdiv(row, g.divN), y = row - z * N;
+  // index c + 1 of a direction component c: the source lies at destination - c, periodic
+  const uint32_t xb[3] = {uint32_t((x == N - 1 ? 0 : x + 1) * sizeof(raw)), uint32_t(x * sizeof(raw)),
+                          uint32_t((x == 0 ? N - 1 : x - 1) * sizeof(raw))};
+  const uint64_t zo[3] = {uint64_t(z == N - 1 ? 0 : z + 1) * g.zs, uint64_t(z) * g.zs, uint64_t(z == 0 ? N - 1 : z - 1) * g.zs};
+  const uint32_t yo[3] = {(y == N - 1 ? 0 : y + 1) * N, y * N, (y == 0 ? N - 1 : y - 1) * N};
+  R fin[Q];
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const char* base = reinterpret_cast<const char*>(fa + (uint64_t(q) * g.qs + zo[CZ(q) + 1] + yo[CY(q) + 1]));
+    const raw* p = reinterpret_cast<const raw*>(base + xb[CX(q) + 1]);
+    if constexpr (kNtLoad) fin[q] = S::template in<q>(__builtin_nontemporal_load(p), dec);
+    else fin[q] = S::template in<q>(*p, dec);
+  });
+  const Collider<S> col(fin);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const raw v = S::template out<q>(col.template post<q>(fin[q], omega), enc);
+    char* base = reinterpret_cast<char*>(fb + (uint64_t(q) * g.qs + zo[1] + yo[1]));
+    raw* p = reinterpret_cast<raw*>(base + xb[1]);
+    if constexpr (kNtStore) __builtin_nontemporal_store(v, p);
+    else *p = v;
+  });
+}
+
 // Macroscopic field, src/tgv3d_uniform.cu:80-90.
 template <class S, int BS>
 __global__ __launch_bounds__(BS) void k_dense_macro(const typename S::raw* __restrict__ f, Dense g, float dec,
@@ -498,13 +536,23 @@ static hipError_t step_v(const void* fa, void* fb, const Dense& g, double omega,
                          int* occ) {
   using R = typename S::real;
   const uint64_t cells = uint64_t(g.N) * g.N * g.N;
+  const bool in_row = g.N % BS == 0;  // every workgroup inside one x-row
   if (occ) {  // workgroups one CU holds at once, instead of a launch
-    if constexpr (V == 1) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step<S, BS, NL, NS>, BS, 0);
-    else return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step_vec<S, BS, V, NL, NS>, BS, 0);
+    if constexpr (V == 1) {
+      if (in_row) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step_row<S, BS, NL, NS>, BS, 0);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step<S, BS, NL, NS>, BS, 0);
+    } else {
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_dense_step_vec<S, BS, V, NL, NS>, BS, 0);
+    }
   }
-  if constexpr (V == 1)
-    hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
-                       static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
+  if constexpr (V == 1) {
+    if (in_row)
+      hipLaunchKernelGGL((k_dense_step_row<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
+                         static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
+    else
+      hipLaunchKernelGGL((k_dense_step<S, BS, NL, NS>), dim3(uint32_t(cells / BS)), dim3(BS), 0, st,
+                         static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);
+  }
   else
     hipLaunchKernelGGL((k_dense_step_vec<S, BS, V, NL, NS>), dim3(uint32_t(cells / (uint64_t(BS) * V))), dim3(BS), 0, st,
                        static_cast<const typename S::raw*>(fa), static_cast<typename S::raw*>(fb), g, R(omega), enc, dec);

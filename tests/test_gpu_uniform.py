@@ -198,11 +198,12 @@ def test_autotune_rejects_amr():
     assert e.value.code == jpabi.JP_ERR_STATE
 
 
-@pytest.mark.parametrize("prec", [jpabi.PREC_FP32, jpabi.PREC_FP16])
-def test_every_single_step_variant_is_bit_exact(prec):
+@pytest.mark.parametrize("prec,n", [(jpabi.PREC_FP32, 32), (jpabi.PREC_FP16, 32), (jpabi.PREC_FP32, 256),
+                                    (jpabi.PREC_FP16, 256), (jpabi.PREC_BF16, 128), (jpabi.PREC_FP64, 128)])
+def test_every_single_step_variant_is_bit_exact(prec, n):
     """The 64 launch shapes of the single step (workgroup size, cache policy, cells per lane) compute the
-    same populations as the oracle."""
-    n = 32
+    same populations as the oracle. Where the workgroup size divides the edge (128 and 256 lanes at 256^3,
+    128 at 128^3), one cell per lane runs the row form, whose loads take scalar row bases."""
     o = orc.Uniform(n, re=1600, prec=prec)
     o.init()
     o.step(2)
