@@ -198,12 +198,9 @@ struct FormF16 {
   template <int q> __device__ static inline raw out(real d, float enc) { return static_cast<_Float16>(d * enc); }
   template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
 };
-__device__ inline uint16_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
+// binary32 -> bfloat16, round to nearest even: one v_cvt_pk_bf16_f32 on gfx950 (the integer form, add
+// 0x7fff plus the kept lsb and truncate, took five instructions per value); NaN stays NaN.
+__device__ inline uint16_t bf16_rne(float x) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x)); }
 struct FormBF16 {
   using raw = uint16_t;
   using real = float;
