@@ -229,6 +229,8 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
     raw v[Q];
     R gv[Q];
     uint32_t from_ghost = 0;
+    // a wave is one z-plane of the tile (lane = cell), so k and everything derived from it is scalar
+    const int kw = __builtin_amdgcn_readfirstlane(k);
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
@@ -236,13 +238,17 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
       if constexpr (q == 0) {
         v[0] = fa[tidx(b, 0, cell)];
       } else {
-        const int si = i - cx, sj = j - cy, sk = k - cz;
+        const int si = i - cx, sj = j - cy, sk = kw - cz;
         const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
         const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
         const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
         const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        const bool same = t >= 0;
-        v[q] = fa[tidx(same ? t : b, q, same ? ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7) : cell)];
+        // a ghost lane reads the same in-tile index of its own tile (in bounds, discarded); the direction's
+        // offset goes into the scalar base, so a lane adds only tile * 19 * 512 + cell (one 64-bit
+        // multiply-add and one shifted add)
+        const raw* fq = fa + q * MB3;
+        const uint32_t src = uint32_t(((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7));
+        v[q] = fq[size_t(uint32_t(t >= 0 ? t : b)) * (Q * MB3) + src];
         if (t < 0) {
           gv[q] = g[ghost_slot(q, i, j, k)];
           from_ghost |= 1u << q;
