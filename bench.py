@@ -99,8 +99,13 @@ def bench_uniform(n, prec, steps, warmup=20, pairs=True, pair_shape=-1):
         g.set_step_pair_shape(pair_shape)
     r = g.bench(steps, warmup=warmup)
     r["pair_shape"] = list(g.step_pair_shape())
+    per_launch = g.steps_per_launch()
     g.close()
-    r["frac_hbm"] = r["gbps"] / PEAK_HBM_GBPS
+    # a fused launch reads and writes the lattice once for per_launch steps: frac_hbm is priced on those
+    # compulsory bytes; gbps stays the rate at bytes_per_update per cell update
+    r["steps_per_launch"] = per_launch
+    r["compulsory_gbps"] = r["gbps"] / per_launch
+    r["frac_hbm"] = r["compulsory_gbps"] / PEAK_HBM_GBPS
     return r
 
 

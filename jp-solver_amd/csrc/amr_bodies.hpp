@@ -115,13 +115,13 @@ __device__ __forceinline__ void collide_store(typename S::real (&fin)[Q], typena
 // = cell, so wave k holds the z-plane k of the tile and a parent's children are a 2x2 lane patch (lane +
 // 0, 1, 8, 9) in waves 2pk and 2pk + 1. The children are summed in the reference's order c = 0..7 (di, dj,
 // dk): the even wave adds its four with cross-lane reads and leaves the partial sum in LDS, the odd wave
-// adds its four and leaves the mean in LDS, and one full wave finishes the 64 parents (a lane each).
-// Call with all 512 lanes.
-// Lane i receives lane i + N of its 16-lane row (DPP row_shl:N, a VALU move with no LDS round trip);
+// adds its four and leaves the sum in LDS, and one full wave finishes the 64 parents (a lane each).
+// The reference sums 0.125 f_c; this sums f_c and scales the total by 0.125 once. Scaling by a power of
+// two is exact and commutes with rounding while values stay in the normal range (populations and their
+// sums are far above 2^-126), so each partial sum is exactly 8 times the reference's and the mean is the
+// same bits. Call with all 512 lanes.
+// Lane i receives lane i + N of its 16-lane row (DPP row_shl:N, VALU moves with no LDS round trip);
 // lanes whose source lies past the row's end keep v.
-template <int N> __device__ __forceinline__ float row_down(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x100 + N, 0xF, 0xF, false));
-}
 template <int N> __device__ __forceinline__ double row_down(double v) {
   const int2 h = *reinterpret_cast<const int2*>(&v);
   int2 r;
@@ -129,9 +129,25 @@ template <int N> __device__ __forceinline__ double row_down(double v) {
   r.y = __builtin_amdgcn_update_dpp(h.y, h.y, 0x100 + N, 0xF, 0xF, false);
   return *reinterpret_cast<const double*>(&r);
 }
+// acc + (v of lane i + N in the 16-lane row); lanes whose source lies past the row's end add 0. The
+// zero-filled cross-lane read folds into the add (one v_add_f32 with a DPP operand).
+template <int N> __device__ __forceinline__ float add_row_down(float acc, float v) {
+  return acc + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + N, 0xF, 0xF, true));
+}
+template <int N> __device__ __forceinline__ double add_row_down(double acc, double v) { return acc + row_down<N>(v); }
+// (((s + v) + v[+1]) + v[+8]) + v[+9]: four children of a parent added to its running sum s, in the
+// reference's child order, at the parent's corner lane (even i, even j; its sources +1, +8, +9 always lie
+// in its own 16-lane row, which holds j = 2m, 2m + 1).
+// The empty asm pins the sum outside the caller's corner-lane branch: sunk into it, the adds would run
+// under a partial exec mask and the cross-lane reads could no longer fold into them.
+template <class R> __device__ __forceinline__ R add_children(R s, R v) {
+  R r = add_row_down<9>(add_row_down<8>(add_row_down<1>(s + v, v), v), v);
+  asm volatile("" : "+v"(r));
+  return r;
+}
 template <class R> struct RestrictLds {
   R part[HALF][Q][16];  // partial sums per plane pair, direction, parent of the pair's 4x4 layer
-  R avg[Q][64];         // complete child means per direction and parent (pk * 16 + pj * 4 + pi)
+  R avg[Q][64];         // complete child sums per direction and parent (pk * 16 + pj * 4 + pi)
 };
 // Parent cell (pi, pj, pk) of a fine tile's octant from its children's mean: feq + nqi * regularize(avg -
 // feq), stored to the parent tile pcb (or the dense level 0).
@@ -166,28 +182,22 @@ __device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q],
   const int k = cell >> 6, lane = cell & 63, i = lane & 7, j = lane >> 3;  // k is uniform in a wave
   const bool corner = !(i & 1) && !(j & 1);
   const int pp = (j >> 1) * 4 + (i >> 1);
-  // a corner lane (even i, even j) reads lanes +1, +8, +9: always in its own 16-lane row (rows hold j = 2m, 2m + 1)
+  R s[Q];
   if (!(k & 1)) {  // children c = 0..3 of plane 2pk
-    sfor<Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const R t = R(0.125) * kept[q];
-      const R t1 = row_down<1>(t), t8 = row_down<8>(t), t9 = row_down<9>(t);
-      if (corner) L.part[k >> 1][q][pp] = ((((R(0) + t) + t1) + t8) + t9);
-    });
+    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(R(0), kept[decltype(qc)::value]); });
+    if (corner) sfor<Q>([&](auto qc) { L.part[k >> 1][decltype(qc)::value][pp] = s[decltype(qc)::value]; });
   }
   __syncthreads();
   if (k & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum
-    sfor<Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const R t = R(0.125) * kept[q];
-      const R t1 = row_down<1>(t), t8 = row_down<8>(t), t9 = row_down<9>(t);
-      if (corner) L.avg[q][(k >> 1) * 16 + pp] = (((L.part[k >> 1][q][pp] + t) + t1) + t8) + t9;
-    });
+    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = R(0); });
+    if (corner) sfor<Q>([&](auto qc) { s[decltype(qc)::value] = L.part[k >> 1][decltype(qc)::value][pp]; });
+    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(s[decltype(qc)::value], kept[decltype(qc)::value]); });
+    if (corner) sfor<Q>([&](auto qc) { L.avg[decltype(qc)::value][(k >> 1) * 16 + pp] = s[decltype(qc)::value]; });
   }
   __syncthreads();
   if (k == 0) {  // one full wave finishes the 64 parents, lane = pk * 16 + pj * 4 + pi
     R avg[Q];
-    sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = L.avg[decltype(qc)::value][lane]; });
+    sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0.125) * L.avg[decltype(qc)::value][lane]; });
     restrict_finish<S, kToDense>(avg, cf, pcb, oct, lane & 3, (lane >> 2) & 3, lane >> 4, G, nqi, cenc);
   }
 }
