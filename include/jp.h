@@ -170,7 +170,7 @@ typedef struct {
  * throughput. Members are reachable with jp_ensemble_member for diagnostics and the test hooks. */
 int jp_ensemble_create(const jp_member* members, int count, int prec, double maxgb, int device, jp_ctx** out);
 /* Tile shape of the batched dense step (not in the reference; a tuning knob that jp_autotune also sets):
- * set = 1 or 8 cells per lane (8 needs every member's lattice to be a multiple of 2048 cells), 0 to only
+ * set = 1 or 8 cells per lane (8 needs uniform members only, each lattice a multiple of 2048 cells), 0 to only
  * query; *current receives the shape in effect. Every shape computes the same result. */
 int jp_ensemble_dense_shape(jp_ctx* ens, int set, int* current);
 /* Borrowed handle of member `index` (owned by the ensemble, valid until jp_destroy of the ensemble). */

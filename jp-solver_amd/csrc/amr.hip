@@ -22,22 +22,17 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step(const ty
     fine_step_cell<S>(fa, fb, gh + size_t(b) * kGhostSlots, snb, om, enc, dec, b, lane);
   });
 }
-// Last fine substep with the restriction into the parent level fused in (ptile/poct: the parent tile
-// and octant of each fine tile, i.e. l2pl1/l2oct or l1pcb/l1oct).
-template <class S, bool kToDense, bool kLoop>
+// Last fine substep with the restriction into the parent level fused in (ptile: the parent tile of each
+// fine tile, l2pl1 or l1pcb); one slab of sibling tiles per unit (fine_step_restrict).
+template <class S, bool kLoop>
 __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                      const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                                      const int* __restrict__ dn, typename S::real om, float enc, float dec,
                                                      typename S::raw* __restrict__ cf, const int* __restrict__ ptile,
-                                                     const int* __restrict__ poct, AmrGeo G, float nqi, float cenc) {
-  __shared__ int snb[kNbr];
+                                                     float nqi, float cenc) {
   __shared__ RestrictLds<typename S::real> L;
-  for_tiles<kLoop>(dn, [=](int b, int lane) {
-    if (lane < kNbr) snb[lane] = nb[size_t(b) * kNbr + lane];
-    __syncthreads();
-    typename S::real kept[Q];
-    fine_step_cell<S, true>(fa, fb, gh + size_t(b) * kGhostSlots, snb, om, enc, dec, b, lane, kept);
-    restrict_tile<S, kToDense>(kept, lane, L, cf, ptile[b], poct[b], G, nqi, cenc);
+  for_tiles<kLoop>(dn, [=](int u, int lane) {
+    fine_step_restrict<S>(fa, fb, gh, nb, ptile, u, lane, om, enc, dec, cf, nqi, cenc, L);
   });
 }
 
@@ -87,14 +82,14 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fill_step_l2(const
 // each launch below carries two independent parts in disjoint workgroup ranges (the first part first in
 // dispatch order): no second stream, no cross-stream wait, and the parts share the GPU. Each part starts
 // at a multiple of 8 workgroups so that workgroup indices keep their XCD.
-//  - k_dense_fill_step_l2: the level-0 step (one cell per lane, nd workgroups) and the fused level-2 fill
-//    and first substep of the pair;
+//  - k_dense_fill_step_l2: the level-0 step (one coarse tile per workgroup, n0 tiles in nd workgroups) and
+//    the fused level-2 fill and first substep of the pair;
 //  - k_step_r_l2_fill_l1: the pair's second level-2 substep with the restriction into level 1 (w2
 //    workgroups) and the level-1 ghost fill from the new level 0.
 template <class S, bool kLoop>
 __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2(const typename S::raw* __restrict__ c0a,
-                                                            typename S::raw* __restrict__ c0b, Dense g0,
-                                                            typename S::real om0, float enc0, float dec0, int nd,
+                                                            typename S::raw* __restrict__ c0b, const int* __restrict__ nb0,
+                                                            int n0, typename S::real om0, float enc0, float dec0, int nd,
                                                             const typename S::raw* __restrict__ f1,
                                                             typename S::real* __restrict__ gh2, const int* __restrict__ l2pl1,
                                                             const int* __restrict__ l2oct, const int* __restrict__ f1pos,
@@ -107,9 +102,12 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
   using R = typename S::real;
   __shared__ FillLds<R> L;
   __shared__ R ghost[kGhostSlots];
-  if (int(blockIdx.x) < nd) {
-    const uint32_t cell = blockIdx.x * 512 + threadIdx.x;
-    if (cell < g0.N * g0.N * g0.N) dense_step_cell<S, false, false>(c0a, c0b, g0, om0, enc0, dec0, cell);
+  if (int(blockIdx.x) < nd) {  // level 0: every neighbour tile exists (periodic), no ghost record is read
+    const int b = xcd_tile(blockIdx.x, nd), lane = threadIdx.x;
+    if (b >= n0) return;
+    if (lane < kNbr) L.nb[lane] = nb0[size_t(b) * kNbr + lane];
+    __syncthreads();
+    fine_step_cell<S>(c0a, c0b, ghost, L.nb, om0, enc0, dec0, b, lane);
     return;
   }
   for_tiles_in<kLoop, false>(n2, dn2, int(blockIdx.x) - nd, int(gridDim.x) - nd, [=](int b, int lane) {
@@ -124,23 +122,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kLoop && si
                                                            const int* __restrict__ nb2, int n2, const int* __restrict__ dn2,
                                                            int w2, typename S::real om2, float enc2, float dec2,
                                                            typename S::raw* __restrict__ f1, const int* __restrict__ l2pl1,
-                                                           const int* __restrict__ l2oct, AmrGeo G, float nqi21, float enc1,
+                                                           AmrGeo G, float nqi21, float enc1,
                                                            const typename S::raw* __restrict__ c0,
                                                            typename S::real* __restrict__ gh1, const int* __restrict__ l1pcb,
                                                            const int* __restrict__ l1oct, const int* __restrict__ nb1,
                                                            const uint32_t* __restrict__ slots, int n1,
                                                            const int* __restrict__ dn1, float nqs01, float dec0) {
   using R = typename S::real;
-  __shared__ int snb[kNbr];
   __shared__ RestrictLds<R> RL;
   __shared__ FillLds<R> L;
   if (int(blockIdx.x) < w2) {
-    for_tiles_in<kLoop>(n2, dn2, blockIdx.x, w2, [=](int b, int lane) {
-      if (lane < kNbr) snb[lane] = nb2[size_t(b) * kNbr + lane];
-      __syncthreads();
-      R kept[Q];
-      fine_step_cell<S, true>(fa, fb, gh2 + size_t(b) * kGhostSlots, snb, om2, enc2, dec2, b, lane, kept);
-      restrict_tile<S, false>(kept, lane, RL, f1, l2pl1[b], l2oct[b], G, nqi21, enc1);
+    for_tiles_in<kLoop>(n2, dn2, blockIdx.x, w2, [=](int u, int lane) {
+      fine_step_restrict<S>(fa, fb, gh2, nb2, l2pl1, u, lane, om2, enc2, dec2, f1, nqi21, enc1, RL);
     });
     return;
   }
@@ -164,7 +157,7 @@ template <class S> __device__ inline void prolong_l0(const typename S::raw* __re
     const int di = c & 1, dj = (c >> 1) & 1, dk = c >> 2;
     const float wgt = (di ? tx : 1 - tx) * (dj ? ty : 1 - ty) * (dk ? tz : 1 - tz);
     R fq[Q], nq[Q];
-    load_dense<S>(c0, G.g0.qs, dense_cell(G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0)), dec, fq);
+    load_l0<S>(c0, G, pmod(ib + di, G.N0), pmod(jb + dj, G.N0), pmod(kb + dk, G.N0), dec, fq);
     const Mom<R> m = moments(fq);
     const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
     sfor<Q>([&](auto qc) {
@@ -284,15 +277,14 @@ __device__ inline float wave_max(float v) {
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
   return v;
 }
-template <class S, bool kDense, bool kLoop>
+template <class S, bool kLoop>
 __global__ __launch_bounds__(512) void k_sensor(const typename S::raw* __restrict__ f, const int* __restrict__ dn, AmrGeo G,
                                                 float dec, float* __restrict__ sen) {
   using R = typename S::real;
   __shared__ float sm[8];
   for_tiles<kLoop>(dn, [=](int b, int cell) {
     R v[Q];
-    if constexpr (kDense) load_dense<S>(f, G.g0.qs, dense_of_tile(G, b, cell), dec, v);
-    else load_tile<S>(f, b, cell, dec, v);
+    load_tile<S>(f, b, cell, dec, v);
     const Mom<R> m = moments(v);
     const EqPre<R> e = eq_pre<R>(m.rho, m.ux / m.rho, m.uy / m.rho, m.uz / m.rho);
     R s = R(0);
@@ -334,7 +326,7 @@ __global__ __launch_bounds__(256) void k_composite(const typename S::raw* __rest
     const int g1i = gi / 2, g1j = gj / 2, g1k = gk / 2;
     load_tile<S>(f1, lb, ((g1k % MB) * MB + (g1j % MB)) * MB + (g1i % MB), dec1, v);
   } else {
-    load_dense<S>(c0, G.g0.qs, dense_cell(G, gi / 4, gj / 4, gk / 4), dec0, v);
+    load_l0<S>(c0, G, gi / 4, gj / 4, gk / 4, dec0, v);
   }
   const Mom<R> m = moments(v);
   V4<R> o;
@@ -380,19 +372,12 @@ hipError_t amr_fine_step(int prec, int, const void* fa, void* fb, const void* gh
   return hipGetLastError();
 }
 
-hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
-                                  const int* nb, Tiles t, double omega, float enc, float dec, void* cf, const int* ptile,
-                                  const int* poct, float nqi, float cenc, hipStream_t st) {
+hipError_t amr_fine_step_restrict(int prec, const void* fa, void* fb, const void* gh, const int* nb, Tiles t, double omega,
+                                  float enc, float dec, void* cf, const int* ptile, float nqi, float cenc, hipStream_t st) {
   if (t.width <= 0) return hipSuccess;
-  if (to_dense) {
-    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step_r<S, true, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb),
-                                         REAL(gh), nb, t.count, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G,
-                                         nqi, cenc)));
-  } else {
-    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step_r<S, false, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb),
-                                         REAL(gh), nb, t.count, typename S::real(omega), enc, dec, WRAW(cf), ptile, poct, G,
-                                         nqi, cenc)));
-  }
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fine_step_r<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(fa), WRAW(fb),
+                                                   REAL(gh), nb, t.count, typename S::real(omega), enc, dec, WRAW(cf), ptile,
+                                                   nqi, cenc)));
   return hipGetLastError();
 }
 
@@ -426,29 +411,28 @@ hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2
 // Part width: the launch width of a Tiles rounded up to a multiple of 8 (one-tile form: the count).
 static inline int part8(Tiles t) { return t.width > 0 ? (t.width + 7) / 8 * 8 : 0; }
 
-hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, double om0, float enc0,
-                                  float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, const int* nb0, double om0,
+                                  float enc0, float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
                                   const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t2,
                                   float nqs12, float dec1, const void* f2a, void* f2b, double om2, float enc2, float dec2,
                                   hipStream_t st) {
-  const long cells = long(G.g0.N) * G.g0.N * G.g0.N;
-  const int nd = int((cells / 512 + 7) / 8 * 8), w2 = part8(t2);
+  const int n0 = G.cgx * G.cgx * G.cgx, nd = (n0 + 7) / 8 * 8, w2 = part8(t2);
   JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_dense_fill_step_l2<S, kLoop>), dim3(nd + w2), dim3(512), 0, st,
-                                                    RAW(c0a), WRAW(c0b), G.g0, typename S::real(om0), enc0, dec0, nd,
+                                                    RAW(c0a), WRAW(c0b), nb0, n0, typename S::real(om0), enc0, dec0, nd,
                                                     RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos, l1grid, nb2, slots, t2.width,
                                                     t2.count, G, nqs12, dec1, RAW(f2a), WRAW(f2b),
                                                     typename S::real(om2), enc2, dec2)));
   return hipGetLastError();
 }
 hipError_t amr_step_r_l2_fill_l1(int prec, const AmrGeo& G, const void* f2a, void* f2b, const void* gh2, const int* nb2,
-                                 Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1, const int* l2oct,
+                                 Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1,
                                  float nqi21, float enc1, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
                                  const int* nb1, const uint32_t* slots, Tiles t1, float nqs01, float dec0, hipStream_t st) {
   const int w2 = part8(t2), w1 = part8(t1);
   if (w1 + w2 == 0) return hipSuccess;
   JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_step_r_l2_fill_l1<S, kLoop>), dim3(w2 + w1), dim3(512), 0, st,
                                                     RAW(f2a), WRAW(f2b), REAL(gh2), nb2, t2.width, t2.count, w2,
-                                                    typename S::real(om2), enc2, dec2, WRAW(f1), l2pl1, l2oct, G, nqi21,
+                                                    typename S::real(om2), enc2, dec2, WRAW(f1), l2pl1, G, nqi21,
                                                     enc1, RAW(c0), WREAL(gh1), l1pcb, l1oct, nb1, slots, t1.width,
                                                     t1.count, nqs01, dec0)));
   return hipGetLastError();
@@ -495,13 +479,45 @@ hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const 
   return hipGetLastError();
 }
 
-hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen, hipStream_t st) {
+hipError_t amr_sensor(int prec, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen, hipStream_t st) {
   if (t.width <= 0) return hipSuccess;
-  if (level == 0) {
-    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_sensor<S, true, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f), t.count, G, dec, sen)));
-  } else {
-    JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_sensor<S, false, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f), t.count, G, dec, sen)));
-  }
+  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_sensor<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f), t.count, G, dec, sen)));
+  return hipGetLastError();
+}
+
+// TGV initial state of level 0, one coarse tile per workgroup (the dense lattice's k_dense_init per cell).
+template <class S>
+__global__ __launch_bounds__(512) void k_l0_init(typename S::raw* __restrict__ f, AmrGeo G, double u0, double kf, double px,
+                                                 double py, double pz, float enc) {
+  const int b = blockIdx.x, cell = threadIdx.x;
+  const int bx = b % G.cgx, by = (b / G.cgx) % G.cgx, bz = b / (G.cgx * G.cgx);
+  const EqPre<typename S::real> e = tgv_equilibrium<typename S::real>(
+      uint32_t(bx * MB + (cell & 7)), uint32_t(by * MB + ((cell >> 3) & 7)), uint32_t(bz * MB + (cell >> 6)), u0, kf, px, py, pz);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    f[tidx(b, q, cell)] = S::template store<q>(feq<q>(e), enc);
+  });
+}
+hipError_t amr_l0_init(int prec, const AmrGeo& G, void* f, double u0, double kf, const double* phase, float enc, hipStream_t st) {
+  const int n0 = G.cgx * G.cgx * G.cgx;
+  JP_DISPATCH(prec, hipLaunchKernelGGL((k_l0_init<S>), dim3(n0), dim3(512), 0, st, WRAW(f), G, u0, kf, phase[0], phase[1], phase[2],
+                                       enc));
+  return hipGetLastError();
+}
+
+// The 27 neighbour tiles of every level-0 tile, periodic ((dz+1)*9 + (dy+1)*3 + dx+1 as for the fine levels).
+__global__ void k_l0_neighbours(int* __restrict__ nb0, int cgx) {
+  const int n0 = cgx * cgx * cgx;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n0 * kNbr) return;
+  const int b = i / kNbr, d = i % kNbr;
+  const int bx = b % cgx, by = (b / cgx) % cgx, bz = b / (cgx * cgx);
+  const int x = (bx + d % 3 - 1 + cgx) % cgx, y = (by + (d / 3) % 3 - 1 + cgx) % cgx, z = (bz + d / 9 - 1 + cgx) % cgx;
+  nb0[i] = (z * cgx + y) * cgx + x;
+}
+hipError_t amr_l0_neighbours(int* nb0, int cgx, hipStream_t st) {
+  const int n = cgx * cgx * cgx * kNbr;
+  hipLaunchKernelGGL(k_l0_neighbours, dim3((n + 255) / 256), dim3(256), 0, st, nb0, cgx);
   return hipGetLastError();
 }
 

@@ -28,14 +28,13 @@ __device__ __forceinline__ void tri(int g, int& base, float& t) {
     t = 0.75f;
   }
 }
-__device__ __forceinline__ uint32_t dense_cell(const AmrGeo& G, int ci, int cj, int ck) {
-  return (uint32_t(ck) * G.N0 + cj) * G.N0 + ci;
-}
-__device__ __forceinline__ uint32_t dense_of_tile(const AmrGeo& G, int cb, int cell) {
-  const int bx = cb % G.cgx, by = (cb / G.cgx) % G.cgx, bz = cb / (G.cgx * G.cgx);
-  return dense_cell(G, bx * MB + (cell & 7), by * MB + ((cell >> 3) & 7), bz * MB + (cell >> 6));
-}
 __device__ __forceinline__ size_t tidx(int b, int q, int cell) { return (size_t(b) * Q + q) * MB3 + cell; }
+// Level 0 is stored tile-major like the fine levels: coarse cell (ci, cj, ck) is cell ((ck & 7) * 8 + (cj &
+// 7)) * 8 + (ci & 7) of tile ((ck >> 3) * cgx + (cj >> 3)) * cgx + (ci >> 3).
+__device__ __forceinline__ int l0_tile(const AmrGeo& G, int ci, int cj, int ck) {
+  return ((ck >> 3) * G.cgx + (cj >> 3)) * G.cgx + (ci >> 3);
+}
+__device__ __forceinline__ int tile_cell(int i, int j, int k) { return ((k & 7) * MB + (j & 7)) * MB + (i & 7); }
 
 
 // The lane index as a value the compiler cannot see through, taken once per iteration of a loop over
@@ -85,13 +84,10 @@ template <class S> __device__ __forceinline__ void load_tile(const typename S::r
     out[q] = S::template load<q>(f[tidx(b, q, cell)], dec);
   });
 }
-// idx: linear cell of AMR level 0, which is stored in direction planes (qs = plane, zs = N^2)
-template <class S> __device__ __forceinline__ void load_dense(const typename S::raw* __restrict__ f, uint64_t plane, uint32_t idx,
-                                                     float dec, typename S::real* out) {
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    out[q] = S::template load<q>(f[uint64_t(q) * plane + idx], dec);
-  });
+// coarse cell (ci, cj, ck) of level 0
+template <class S> __device__ __forceinline__ void load_l0(const typename S::raw* __restrict__ f, const AmrGeo& G, int ci, int cj,
+                                                  int ck, float dec, typename S::real* out) {
+  load_tile<S>(f, l0_tile(G, ci, cj, ck), tile_cell(ci, cj, ck), dec, out);
 }
 
 // BGK collision of pulled values (the form's working representation), stored to tile b
@@ -109,17 +105,12 @@ __device__ __forceinline__ void collide_store(typename S::real (&fin)[Q], typena
   });
 }
 
-// Restriction of one tile's 64 parent cells from the tile's just-stored populations, fused into the fine
-// step (src/tgv3d_amr.cu:275-298, 358-381): parent = feq + nqi * regularize(mean of the 8 children - feq).
-// The 8 children of a parent cell lie in this tile, so no second pass over the fine level is needed. Lane
-// = cell, so wave k holds the z-plane k of the tile and a parent's children are a 2x2 lane patch (lane +
-// 0, 1, 8, 9) in waves 2pk and 2pk + 1. The children are summed in the reference's order c = 0..7 (di, dj,
-// dk): the even wave adds its four with cross-lane reads and leaves the partial sum in LDS, the odd wave
-// adds its four and leaves the sum in LDS, and one full wave finishes the 64 parents (a lane each).
-// The reference sums 0.125 f_c; this sums f_c and scales the total by 0.125 once. Scaling by a power of
-// two is exact and commutes with rounding while values stay in the normal range (populations and their
-// sums are far above 2^-126), so each partial sum is exactly 8 times the reference's and the mean is the
-// same bits. Call with all 512 lanes.
+// Restriction from the just-stored populations of fine tiles, fused into the last fine substep
+// (src/tgv3d_amr.cu:275-298, 358-381): parent = feq + nqi * regularize(mean of the 8 children - feq). The
+// children are summed in the reference's order c = 0..7 (di, dj, dk). The reference sums 0.125 f_c; this
+// sums f_c and scales the total by 0.125 once. Scaling by a power of two is exact and commutes with
+// rounding while values stay in the normal range (populations and their sums are far above 2^-126), so
+// each partial sum is exactly 8 times the reference's and the mean is the same bits.
 // Lane i receives lane i + N of its 16-lane row (DPP row_shl:N, VALU moves with no LDS round trip);
 // lanes whose source lies past the row's end keep v.
 template <int N> __device__ __forceinline__ double row_down(double v) {
@@ -145,15 +136,10 @@ template <class R> __device__ __forceinline__ R add_children(R s, R v) {
   asm volatile("" : "+v"(r));
   return r;
 }
-template <class R> struct RestrictLds {
-  R part[HALF][Q][16];  // partial sums per plane pair, direction, parent of the pair's 4x4 layer
-  R avg[Q][64];         // complete child sums per direction and parent (pk * 16 + pj * 4 + pi)
-};
-// Parent cell (pi, pj, pk) of a fine tile's octant from its children's mean: feq + nqi * regularize(avg -
-// feq), stored to the parent tile pcb (or the dense level 0).
-template <class S, bool kToDense>
-__device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], typename S::raw* __restrict__ cf, int pcb, int oct, int pi,
-                                       int pj, int pk, const AmrGeo& G, float nqi, float cenc) {
+// Parent cell pc of tile pcb from its children's mean: feq + nqi * regularize(avg - feq).
+template <class S>
+__device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], typename S::raw* __restrict__ cf, int pcb, int pc,
+                                       float nqi, float cenc) {
   using R = typename S::real;
   R nq[Q];
   const Mom<R> m = moments(avg);
@@ -164,42 +150,10 @@ __device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], type
     nq[q] = avg[q] - feq<q>(e);
   });
   regularize(nq);
-  const int lci = 4 * (oct & 1) + pi, lcj = 4 * ((oct >> 1) & 1) + pj, lck = 4 * (oct >> 2) + pk;
-  const int pc = (lck * MB + lcj) * MB + lci;
-  const uint32_t di = kToDense ? dense_of_tile(G, pcb, pc) : 0;
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const typename S::raw v = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
-    if constexpr (kToDense) cf[uint64_t(q) * G.g0.qs + di] = v;
-    else cf[tidx(pcb, q, pc)] = v;
+    cf[tidx(pcb, q, pc)] = S::template store<q>(feq<q>(e) + R(nqi) * nq[q], cenc);
   });
-}
-template <class S, bool kToDense>
-__device__ __forceinline__ void restrict_tile(const typename S::real (&kept)[Q], int cell, RestrictLds<typename S::real>& L,
-                                     typename S::raw* __restrict__ cf, int pcb, int oct, const AmrGeo& G, float nqi,
-                                     float cenc) {
-  using R = typename S::real;
-  const int k = cell >> 6, lane = cell & 63, i = lane & 7, j = lane >> 3;  // k is uniform in a wave
-  const bool corner = !(i & 1) && !(j & 1);
-  const int pp = (j >> 1) * 4 + (i >> 1);
-  R s[Q];
-  if (!(k & 1)) {  // children c = 0..3 of plane 2pk
-    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(R(0), kept[decltype(qc)::value]); });
-    if (corner) sfor<Q>([&](auto qc) { L.part[k >> 1][decltype(qc)::value][pp] = s[decltype(qc)::value]; });
-  }
-  __syncthreads();
-  if (k & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum
-    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = R(0); });
-    if (corner) sfor<Q>([&](auto qc) { s[decltype(qc)::value] = L.part[k >> 1][decltype(qc)::value][pp]; });
-    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(s[decltype(qc)::value], kept[decltype(qc)::value]); });
-    if (corner) sfor<Q>([&](auto qc) { L.avg[decltype(qc)::value][(k >> 1) * 16 + pp] = s[decltype(qc)::value]; });
-  }
-  __syncthreads();
-  if (k == 0) {  // one full wave finishes the 64 parents, lane = pk * 16 + pj * 4 + pi
-    R avg[Q];
-    sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0.125) * L.avg[decltype(qc)::value][lane]; });
-    restrict_finish<S, kToDense>(avg, cf, pcb, oct, lane & 3, (lane >> 2) & 3, lane >> 4, G, nqi, cenc);
-  }
 }
 
 // Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
@@ -286,6 +240,58 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
   fine_step_cell<S, false>(fa, fb, g, snb, om, enc, dec, b, cell, none);
 }
 
+// Last fine substep with the restriction fused in, over one slab of sibling tiles. The 8 fine tiles of a
+// parent tile are 8r + oct (oct = qz * 4 + qy * 2 + qx: the octant they refine), and unit u of a launch over
+// n tiles (n a multiple of 8) takes the four siblings 8r + 4qz + s (s = qy * 2 + qx) and their z-planes 2pk
+// and 2pk + 1, with r = u / 8, qz = (u / 4) % 2, pk = u % 4: wave w steps plane 2pk + w % 2 of sibling w / 2.
+// These planes hold all the children of the 8 x 8 parent cells of layer qz * 4 + pk of the parent tile.
+// The even wave of a sibling adds each parent's four children of plane 2pk (lanes +0, +1, +8, +9 of a
+// corner lane) and leaves the partial sum in LDS, the odd wave adds the four of plane 2pk + 1, and one wave
+// finishes the 64 parents, a lane each in x-fastest order, so each direction is stored as one 8 x 8 block of
+// 256 contiguous bytes of the (tile-major) parent level. One tile's own octant, 4 x 4 x 4 parents, would
+// store 16 rows of 16 B per direction.
+// ptile: the parent tile of each fine tile (l1pcb or l2pl1). Call with all 512 lanes.
+template <class R> struct RestrictLds {
+  int nb[4 * kNbr];     // the four siblings' neighbour entries
+  R part[4][Q][16];     // partial sums per sibling, direction, parent of the sibling's 4 x 4 layer
+  R avg[Q][64];         // complete child sums per direction and parent (py * 8 + px)
+};
+template <class S>
+__device__ __forceinline__ void fine_step_restrict(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+                                          const typename S::real* __restrict__ gh, const int* __restrict__ nb,
+                                          const int* __restrict__ ptile, int u, int lane, typename S::real om, float enc,
+                                          float dec, typename S::raw* __restrict__ cf, float nqi, float cenc,
+                                          RestrictLds<typename S::real>& L) {
+  using R = typename S::real;
+  const int t0 = (u >> 3) * 8 + ((u >> 2) & 1) * 4, pz = ((u >> 2) & 1) * 4 + (u & 3);
+  const int w = lane >> 6, l = lane & 63, sib = w >> 1;  // w and sib are uniform in a wave
+  if (lane < 4 * kNbr) L.nb[lane] = nb[size_t(t0) * kNbr + lane];
+  __syncthreads();
+  const int b = t0 + sib, cell = (2 * (u & 3) + (w & 1)) * 64 + l;
+  R kept[Q];
+  fine_step_cell<S, true>(fa, fb, gh + size_t(b) * kGhostSlots, L.nb + sib * kNbr, om, enc, dec, b, cell, kept);
+  const int i = l & 7, j = l >> 3;
+  const bool corner = !(i & 1) && !(j & 1);
+  const int pp = (j >> 1) * 4 + (i >> 1);
+  R s[Q];
+  if (!(w & 1)) {  // children c = 0..3 of plane 2pk
+    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(R(0), kept[decltype(qc)::value]); });
+    if (corner) sfor<Q>([&](auto qc) { L.part[sib][decltype(qc)::value][pp] = s[decltype(qc)::value]; });
+  }
+  __syncthreads();
+  if (w & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum (every lane reads its parent's)
+    const int pl = ((sib >> 1) * 4 + (j >> 1)) * 8 + (sib & 1) * 4 + (i >> 1);
+    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(L.part[sib][decltype(qc)::value][pp], kept[decltype(qc)::value]); });
+    if (corner) sfor<Q>([&](auto qc) { L.avg[decltype(qc)::value][pl] = s[decltype(qc)::value]; });
+  }
+  __syncthreads();
+  if (w == 0) {  // one full wave finishes the 64 parents of layer pz, lane = py * 8 + px
+    R avg[Q];
+    sfor<Q>([&](auto qc) { avg[decltype(qc)::value] = R(0.125) * L.avg[decltype(qc)::value][l]; });
+    restrict_finish<S>(avg, cf, ptile[t0], pz * 64 + l, nqi, cenc);
+  }
+}
+
 // prep = feq + nqs * regularize(f - feq) of one cell, in place (src/tgv3d_amr.cu:160-171).
 template <class R> __device__ __forceinline__ void prep_values(R* v, float nqs) {
   R nq[Q];
@@ -360,7 +366,7 @@ __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
     R v[Q];
-    load_dense<S>(c0, G.g0.qs, dense_cell(G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0)), dec, v);
+    load_l0<S>(c0, G, pmod(bx0 + li, G.N0), pmod(by0 + lj, G.N0), pmod(bz0 + lk, G.N0), dec, v);
     prep_values(v, nqs);
     sfor<Q>([&](auto qc) { L.box[decltype(qc)::value * kBox3 + c] = v[decltype(qc)::value]; });
   }

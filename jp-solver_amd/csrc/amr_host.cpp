@@ -17,7 +17,6 @@ namespace jp {
 
 static AmrGeo geo(const jp_ctx* c) {
   AmrGeo G;
-  G.g0 = c->g0;
   G.N0 = c->amr.N;
   G.cgx = c->amr.cgx;
   G.l1gx = c->amr.l1gx;
@@ -54,6 +53,13 @@ static Items items(const jp_ctx* c, int level) {
   return {(a.resident || a.dev_counts) ? (level == 1 ? a.w1 : a.w2) : (level == 1 ? a.nf1 : a.nf2), dn};
 }
 static Tiles tiles0(const jp_ctx* c) { return {nullptr, c->amr.cnb0}; }
+// One level-0 step c0a -> c0b (the fine-level step over all cnb0 tiles; no ghost record is read).
+static int l0_step(jp_ctx* c, hipStream_t st) {
+  HCK(c, amr_fine_step(c->prec, 0, c->c0a, c->c0b, nullptr, c->amr.nb0, tiles0(c), c->d.omega0, float(c->d.scale[0]),
+                       float(1.0 / c->d.scale[0]), st));
+  std::swap(c->c0a, c->c0b);
+  return JP_OK;
+}
 
 // Fill table of the compact ghost record, one entry per slot (encoding in amr_internal.hpp). Every fine
 // tile's origin is even on its own grid, so the box offset and the weights depend on the slot only.
@@ -255,7 +261,7 @@ static int upload_sensor(jp_ctx* c, const float* sen0, int n) {
 }
 static int sensor_l1(jp_ctx* c, const float* sen1_by_pos) {
   AmrState& a = c->amr;
-  HCK(c, amr_sensor(c->prec, 1, geo(c), a.f1a, tiles(c, 1), dec(c, 1), a.sen1, c->st));
+  HCK(c, amr_sensor(c->prec, geo(c), a.f1a, tiles(c, 1), dec(c, 1), a.sen1, c->st));
   if (sen1_by_pos) {
     HCK(c, hipMemcpyAsync(a.sen1pos, sen1_by_pos, size_t(a.NG1) * 4, hipMemcpyHostToDevice, c->st));
     HCK(c, rg_gather_by_pos(a.sen1pos, a.f1pos, items(c, 1), a.sen1, c->st));
@@ -271,18 +277,16 @@ int amr_init_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   drop_graph(c);
-  HCK(c, dense_init(c->prec, c->c0a, c->g0, c->u0, c->d.kf, c->phase, enc(c, 0), c->st));
+  HCK(c, amr_l0_init(c->prec, G, c->c0a, c->u0, c->d.kf, c->phase, enc(c, 0), c->st));
   if (int r = fill(c, a.dcnt, 0, CNT_N)) return r;
   a.hw_nf1 = a.hw_nf2 = 0;
   a.counts_stale = false;
   if (a.l1_mode == JP_MODE_ALL) HCK(c, rg_set(a.ref0, a.cnb0, 0, a.cgx, nullptr, c->st));
   else if (a.l1_mode == JP_MODE_SLAB) HCK(c, rg_set(a.ref0, a.cnb0, 1, a.cgx, nullptr, c->st));
   else {
-    for (int s = 0; s < 300; ++s) {  // develop(300): unrecorded coarse steps before the first marking
-      HCK(c, dense_step(c->prec, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, c->st));
-      std::swap(c->c0a, c->c0b);
-    }
-    HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
+    for (int s = 0; s < 300; ++s)  // develop(300): unrecorded coarse steps before the first marking
+      if (int r = l0_step(c, c->st)) return r;
+    HCK(c, amr_sensor(c->prec, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
     if (sen0)
       if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
     HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
@@ -355,7 +359,7 @@ static int regrid_device(jp_ctx* c, bool clamp) {
   AmrState& a = c->amr;
   const AmrGeo G = geo(c);
   if (int r = snapshot_tables(c)) return r;
-  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
+  HCK(c, amr_sensor(c->prec, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
   HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
   HCK(c, rg_mark_thr(a.sen0, a.thr, a.ref0, a.cnb0, c->st));
   HCK(c, rg_scan(a.ref0, {a.cnb0, nullptr}, a.rank, a.dcnt, CNT_NREF0, CNT_NF1, a.cap1, c->st));
@@ -403,7 +407,7 @@ int amr_regrid_with(jp_ctx* c, const float* sen0, const float* sen1_by_pos) {
   }
   if (snapshot)
     if (int r = snapshot_tables(c)) return r;
-  HCK(c, amr_sensor(c->prec, 0, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
+  HCK(c, amr_sensor(c->prec, G, c->c0a, tiles0(c), dec(c, 0), a.sen0, c->st));
   if (sen0)
     if (int r = upload_sensor(c, sen0, a.cnb0)) return r;
   HCK(c, rg_select(a.sen0, nullptr, {a.cnb0, nullptr}, a.frac1, a.cap_nref0, a.thr, a.dcnt, CNT_NSEL0, c->st));
@@ -560,8 +564,8 @@ static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
   if (a.skiprestr)
     HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
   else
-    HCK(c, amr_fine_step_restrict(p, false, G, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a,
-                                  a.l2pl1, a.l2oct, float(c->d.nqi21), enc(c, 1), st));
+    HCK(c, amr_fine_step_restrict(p, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a, a.l2pl1,
+                                  float(c->d.nqi21), enc(c, 1), st));
   std::swap(a.f2a, a.f2b);
   return JP_OK;
 }
@@ -582,14 +586,14 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
   const bool l2 = a.resident || a.nf2 > 0;
   const Tiles t1 = tiles(c, 1), t2 = tiles(c, 2);
   if (l2 && !a.skipprep && !a.skiprestr) {
-    HCK(c, amr_dense_fill_step_l2(p, G, c->c0a, c->c0b, c->d.omega0, enc(c, 0), dec(c, 0), a.f1a, a.gh2, a.l2pl1, a.l2oct,
+    HCK(c, amr_dense_fill_step_l2(p, G, c->c0a, c->c0b, a.nb0, c->d.omega0, enc(c, 0), dec(c, 0), a.f1a, a.gh2, a.l2pl1, a.l2oct,
                                   a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12), dec(c, 1), a.f2a, a.f2b,
                                   c->d.omega2, enc(c, 2), dec(c, 2), st));
     std::swap(c->c0a, c->c0b);
     std::swap(a.f2a, a.f2b);
     // prep of level 0 (src/tgv3d_amr.cu:826) is computed inside the fill
     HCK(c, amr_step_r_l2_fill_l1(p, G, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), a.f1a, a.l2pl1,
-                                 a.l2oct, float(c->d.nqi21), enc(c, 1), c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1,
+                                 float(c->d.nqi21), enc(c, 1), c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1,
                                  float(c->d.nqs01), dec(c, 0), st));
     std::swap(a.f2a, a.f2b);
   } else {
@@ -601,8 +605,7 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
       if (int r = level2_pair(c, G, a.st2)) return r;
       HCK(c, hipEventRecord(ev_join, a.st2));
     }
-    HCK(c, dense_step(p, c->c0a, c->c0b, c->g0, c->d.omega0, enc(c, 0), dec(c, 0), c->variant, st));
-    std::swap(c->c0a, c->c0b);
+    if (int r = l0_step(c, st)) return r;
     if (!a.resident && !a.nf1) return JP_OK;
     if (!a.skipprep)
       HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), st));
@@ -615,8 +618,8 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
   if (a.skiprestr)
     HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));
   else  // the second level-1 substep restricts into level 0
-    HCK(c, amr_fine_step_restrict(p, true, G, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a,
-                                  a.l1pcb, a.l1oct, float(c->d.nqi10), enc(c, 0), st));
+    HCK(c, amr_fine_step_restrict(p, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), c->c0a, a.l1pcb,
+                                  float(c->d.nqi10), enc(c, 0), st));
   std::swap(a.f1a, a.f1b);
   return JP_OK;
 }
@@ -682,15 +685,13 @@ int amr_create(const jp_amr_params* p, jp_ctx* c) {
     HCK(c, hipEventCreateWithFlags(&a.ev_join[i], hipEventDisableTiming));
   }
   // pools
-  // Level 0 in direction planes (the interface kernels address it by linear cell), 32 elements after each:
-  // at N0 = 128 the 8 MiB power-of-two plane stride cost the AMR coarse step 3 % (0.896 against 0.868 ms);
-  // at N0 = 96 the pad changes nothing measurable.
-  c->g0 = make_dense(N, 32);
-  c->variant = default_variant(c->prec);
+  // Level 0 tile-major: cnb0 tiles of 19 x 512 elements, and its periodic neighbour table
   const size_t e = prec_bytes(c->prec);
-  const size_t dense_b = size_t(c->g0.elems);
+  const size_t dense_b = size_t(a.cnb0) * Q * MB3;
   if (int rc = ctx_alloc(c, &c->c0a, dense_b * e, "c0a")) return rc;
   if (int rc = ctx_alloc(c, &c->c0b, dense_b * e, "c0b")) return rc;
+  if (int rc = ctx_alloc_t(c, &a.nb0, size_t(a.cnb0) * kNbr, "nb0")) return rc;
+  HCK(c, amr_l0_neighbours(a.nb0, a.cgx, c->st));
   if (int rc = alloc_l1_pool(c, true)) return rc;
   // level-1 tables are sized for full refinement (8 cnb0 tiles) so that only the pools ever grow
   const size_t n1 = size_t(8) * a.cnb0, n01 = n1;

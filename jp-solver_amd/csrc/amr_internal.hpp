@@ -1,7 +1,8 @@
 // Launch interface of the AMR kernels (amr.hip, regrid.hip) used by the orchestration in amr_host.cpp.
 //
-// Level 0 is the dense lattice (planes, see internal.hpp). Levels 1 and 2 are scattered tiles of 8^3
-// cells stored tile-major. Each fine tile keeps a compact ghost record of the 1824 (q, cell) pairs whose
+// Level 0 is the coarse lattice, stored tile-major in 8^3 tiles like the fine levels (all cgx^3 tiles, in
+// raster order): a restriction then stores its parents as whole 256 B blocks, where direction planes
+// would take 32 B rows. Levels 1 and 2 are scattered tiles of 8^3 cells stored tile-major. Each fine tile keeps a compact ghost record of the 1824 (q, cell) pairs whose
 // pull source lies outside the tile (64 per axis direction, 120 per diagonal direction), instead of the
 // reference's 19 x 1000 ghost box (src/tgv3d_amr.cu:179-185): the step reads exactly those entries.
 #pragma once
@@ -44,7 +45,6 @@ __host__ __device__ inline int ghost_slot(int q, int i, int j, int k) {
 
 // Geometry shared by the AMR kernels.
 struct AmrGeo {
-  Dense g0;       // level-0 planes
   int N0, cgx, l1gx, N1, N2;
 };
 
@@ -60,10 +60,14 @@ struct Tiles {
 // ---- amr.hip ----
 hipError_t amr_fine_step(int prec, int level, const void* fa, void* fb, const void* gh, const int* nb, Tiles t,
                          double omega, float enc, float dec, hipStream_t st);
-// fine step + restriction into the parent level (dense level 0 when to_dense)
-hipError_t amr_fine_step_restrict(int prec, bool to_dense, const AmrGeo& G, const void* fa, void* fb, const void* gh,
-                                  const int* nb, Tiles t, double omega, float enc, float dec, void* cf, const int* ptile,
-                                  const int* poct, float nqi, float cenc, hipStream_t st);
+// fine step + restriction into the parent level cf (ptile: l1pcb or l2pl1); t covers whole octets of tiles
+hipError_t amr_fine_step_restrict(int prec, const void* fa, void* fb, const void* gh, const int* nb, Tiles t, double omega,
+                                  float enc, float dec, void* cf, const int* ptile, float nqi, float cenc, hipStream_t st);
+// Level 0: TGV initial state, and the periodic 27-neighbour table that its step (amr_fine_step with gh
+// null) reads.
+hipError_t amr_l0_init(int prec, const AmrGeo& G, void* f, double u0, double kf, const double* phase, float enc,
+                       hipStream_t st);
+hipError_t amr_l0_neighbours(int* nb0, int cgx, hipStream_t st);
 // Ghost records of level 1 (from the dense level 0 c0) and level 2 (from level 1 f1): prep values of
 // the parent box computed on the fly, trilinear interpolation of the slots whose source tile is absent.
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
@@ -80,13 +84,13 @@ hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2
 // level-0 step (c0a -> c0b) with the fused level-2 fill and first substep (f2a -> f2b), then the second
 // level-2 substep with its restriction into level 1 (f2a -> f2b, into f1) with the level-1 ghost fill
 // from the new level 0 (c0).
-hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, double om0, float enc0,
-                                  float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
+hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, const int* nb0, double om0,
+                                  float enc0, float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
                                   const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t2,
                                   float nqs12, float dec1, const void* f2a, void* f2b, double om2, float enc2, float dec2,
                                   hipStream_t st);
 hipError_t amr_step_r_l2_fill_l1(int prec, const AmrGeo& G, const void* f2a, void* f2b, const void* gh2, const int* nb2,
-                                 Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1, const int* l2oct,
+                                 Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1,
                                  float nqi21, float enc1, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
                                  const int* nb1, const uint32_t* slots, Tiles t1, float nqs01, float dec0, hipStream_t st);
 hipError_t amr_seed_l1(int prec, const AmrGeo& G, const void* c0, void* f1, const int* l1pcb, const int* l1oct,
@@ -104,8 +108,7 @@ hipError_t amr_migrate_l2(int prec, const AmrGeo& G, const void* oldf2, void* ne
 hipError_t amr_composite_field(int prec, const AmrGeo& G, const void* c0, const void* f1, const void* f2,
                                const int* ref1, const int* fof1, const int* l1grid, const float* dec3, void* F,
                                hipStream_t st);
-hipError_t amr_sensor(int prec, int level, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen,
-                      hipStream_t st);
+hipError_t amr_sensor(int prec, const AmrGeo& G, const void* f, Tiles t, float dec, float* sen, hipStream_t st);
 
 // ---- regrid.hip ----
 // Device counters (int slots of AmrState::dcnt).

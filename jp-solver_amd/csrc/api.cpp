@@ -593,7 +593,7 @@ int jp_get_populations(jp_ctx* c, int level, void* dst, size_t bytes) {
   HCK(c, hipSetDevice(c->device));
   void* src;
   if (int r = level_buffer(c, level, &src)) return r;
-  if (level == 0) {
+  if (level == 0 && c->kind == 0) {  // the uniform lattice's planes or z-layers; AMR level 0 is tile-major
     void* tmp;
     if (int r = ctx_alloc(c, &tmp, need, "tile staging")) return r;
     hipError_t e = dense_to_tiles(c->prec, src, c->g0, tmp, c->st);
@@ -619,7 +619,7 @@ int jp_set_populations(jp_ctx* c, int level, const void* srcp, size_t bytes) {
   void* dstb;
   if (int r = level_buffer(c, level, &dstb)) return r;
   drop_graph(c);
-  if (level == 0) {
+  if (level == 0 && c->kind == 0) {
     void* tmp;
     if (int r = ctx_alloc(c, &tmp, need, "tile staging")) return r;
     hipError_t e = hipMemcpyAsync(tmp, srcp, need, hipMemcpyHostToDevice, c->st);

@@ -40,6 +40,7 @@ struct AmrState {
   int *skin0 = nullptr, *skin1 = nullptr, *o_ref0 = nullptr, *o_fof0 = nullptr, *o_l1grid = nullptr;
   int *o_ref1 = nullptr, *o_fof1 = nullptr, *rank = nullptr, *deep = nullptr, *flags = nullptr;
   int *nb1 = nullptr, *nb2 = nullptr;  // 27 same-level neighbour tiles per fine tile (-1 = absent)
+  int* nb0 = nullptr;                  // 27 neighbour tiles per level-0 tile (periodic, all present)
   float *sen0 = nullptr, *sen1 = nullptr, *sen1pos = nullptr, *thr = nullptr;
   uint32_t* slots = nullptr;  // ghost fill table
   int* dcnt = nullptr;  // device-resident counts and flags (see amr.hip)
@@ -59,7 +60,7 @@ struct jp_ctx {
   jp_derived d{};
   double u0 = 0.05, re = 1600, phase[3] = {0, 0, 0};
   jp::Dense g0{};
-  void *c0a = nullptr, *c0b = nullptr;  // dense level (uniform lattice or AMR level 0)
+  void *c0a = nullptr, *c0b = nullptr;  // dense lattice (g0), or AMR level 0 (tile-major, cnb0 tiles)
   void* field = nullptr;
   size_t field_bytes = 0;
   float* out = nullptr;  // output-field arrays (Q, speed, velocity) of the last jp_field

@@ -5,6 +5,18 @@
 
 namespace jp {
 
+// Equilibrium factors of the TGV initial state at cell (x, y, z) (src/tgv3d_uniform.cu:65-77;
+// src/tgv3d_amr.cu:431-440): FP64 trigonometry, the working type's equilibrium.
+template <class R> __device__ inline EqPre<R> tgv_equilibrium(uint32_t x, uint32_t y, uint32_t z, double u0, double kf, double px,
+                                                              double py, double pz) {
+  const double X = (double(x) + 0.5) * kf + px, Y = (double(y) + 0.5) * kf + py, Z = (double(z) + 0.5) * kf + pz;
+  const double ux = u0 * sin(X) * cos(Y) * cos(Z);
+  const double uy = -u0 * cos(X) * sin(Y) * cos(Z);
+  const double p = (u0 * u0 / 16.0) * (cos(2 * X) + cos(2 * Y)) * (cos(2 * Z) + 2.0);
+  const double rho = 1.0 + 3.0 * p;
+  return eq_pre<R>(R(rho), R(ux), R(uy), R(0));
+}
+
 // Source coordinate arrays for c in {-1, 0, +1} (index c + 1): source = destination - c, periodic; zs are
 // offsets in an x-fastest N^3 field (the macro field), not in the population layout.
 struct Nbr {

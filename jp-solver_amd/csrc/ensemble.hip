@@ -15,16 +15,27 @@ template <class S> __device__ inline const typename S::raw* cr(const void* p) {
 }
 template <class S> __device__ inline typename S::raw* wr(void* p) { return static_cast<typename S::raw*>(p); }
 
-// items: (member, 256-cell chunk)
+// items: (member, 256-cell chunk). An AMR member's level 0 is tile-major: chunk k is half k % 2 of tile
+// k / 2, stepped as the fine levels are (every neighbour tile exists, no ghost record is read).
 template <class S>
 __global__ __launch_bounds__(256) void kb_dense_step(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
                                                      int n, int p0) {
   using R = typename S::real;
+  __shared__ int snb[kNbr];
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
-    dense_step_cell<S, false, true>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
-                       uint32_t(w.y) * 256 + threadIdx.x);
+    if (d.nb0) {
+      const int b = w.y >> 1, lane = opaque_lane();
+      __syncthreads();  // the previous item's lanes are done with snb
+      if (lane < kNbr) snb[lane] = d.nb0[size_t(b) * kNbr + lane];
+      __syncthreads();
+      fine_step_cell<S>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), nullptr, snb, R(d.om[0]), d.enc[0], d.dec[0], b,
+                        (w.y & 1) * 256 + lane);
+    } else {
+      dense_step_cell<S, false, true>(cr<S>(d.c0[p0]), wr<S>(d.c0[p0 ^ 1]), d.g0, R(d.om[0]), d.enc[0], d.dec[0],
+                                      uint32_t(w.y) * 256 + threadIdx.x);
+    }
   }
 }
 // items: (member, 2048-cell chunk): 8 consecutive x-cells per lane, one vector load per direction (the
@@ -81,31 +92,24 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void kb_fine_step(const M
   }
 }
 // Second substep of a fine level with the restriction into its parent fused in (the single-context
-// k_fine_step_r): items (member, fine tile); pf: the fine level's parity, pc: the parent level's.
+// k_fine_step_r): items (member, slab unit of fine_step_restrict); pf: the fine level's parity, pc: the
+// parent level's.
 template <class S, int kLevel>
 __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void kb_fine_step_r(const MemberDesc* __restrict__ md, const int2* __restrict__ items,
                                                       int n, int pf, int pc) {
   using R = typename S::real;
-  __shared__ int snb[kNbr];
   __shared__ RestrictLds<R> L;
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const MemberDesc& d = md[w.x];
-    const int* nb = kLevel == 1 ? d.nb1 : d.nb2;
     const int lane = opaque_lane();
-    __syncthreads();  // the previous item's lanes are done with snb and the partial sums
-    if (lane < kNbr) snb[lane] = nb[size_t(w.y) * kNbr + lane];
-    __syncthreads();
-    R kept[Q];
-    if constexpr (kLevel == 1) {
-      fine_step_cell<S, true>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1) + size_t(w.y) * kGhostSlots, snb, R(d.om[1]),
-                              d.enc[1], d.dec[1], w.y, lane, kept);
-      restrict_tile<S, true>(kept, lane, L, wr<S>(d.c0[pc]), d.l1pcb[w.y], d.l1oct[w.y], d.G, d.nqi10, d.enc[0]);
-    } else {
-      fine_step_cell<S, true>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2) + size_t(w.y) * kGhostSlots, snb, R(d.om[2]),
-                              d.enc[2], d.dec[2], w.y, lane, kept);
-      restrict_tile<S, false>(kept, lane, L, wr<S>(d.f1[pc]), d.l2pl1[w.y], d.l2oct[w.y], d.G, d.nqi21, d.enc[1]);
-    }
+    __syncthreads();  // the previous item's lanes are done with the neighbour entries and the partial sums
+    if constexpr (kLevel == 1)
+      fine_step_restrict<S>(cr<S>(d.f1[pf]), wr<S>(d.f1[pf ^ 1]), static_cast<const R*>(d.gh1), d.nb1, d.l1pcb, w.y, lane,
+                            R(d.om[1]), d.enc[1], d.dec[1], wr<S>(d.c0[pc]), d.nqi10, d.enc[0], L);
+    else
+      fine_step_restrict<S>(cr<S>(d.f2[pf]), wr<S>(d.f2[pf ^ 1]), static_cast<const R*>(d.gh2), d.nb2, d.l2pl1, w.y, lane,
+                            R(d.om[2]), d.enc[2], d.dec[2], wr<S>(d.f1[pc]), d.nqi21, d.enc[1], L);
   }
 }
 

@@ -13,9 +13,9 @@ struct MemberDesc {
   void* f2[2];
   void *gh1, *gh2;
   const int *ref0, *fof0, *rl0, *l1pcb, *l1oct, *f1pos, *l1grid, *ref1, *fof1, *rl1, *l2pl1, *l2oct;
-  const int *nb1, *nb2;
+  const int *nb0, *nb1, *nb2;  // nb0: an AMR member's level-0 neighbour table (null for a uniform member)
   const uint32_t* slots;
-  Dense g0;
+  Dense g0;  // a uniform member's lattice
   AmrGeo G;
   double om[3];
   float enc[3], dec[3];

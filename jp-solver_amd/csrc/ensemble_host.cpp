@@ -36,7 +36,7 @@ struct EnsState {
 };
 
 // Workgroups of a batched launch. A workgroup of the fused step + restriction kernels ends every item with
-// one wave finishing the tile's 64 parents while the others wait, so a workgroup walking several items
+// one wave finishing 64 parents while the others wait, so a workgroup walking several items
 // would serialise those tails; with one workgroup per item the dispatcher overlaps them with other
 // items. The other kernels take their autotuned count.
 static int ens_groups(const EnsState& e, int k) {
@@ -69,11 +69,11 @@ static MemberDesc describe(const jp_ctx* m) {
   d.rl1 = a.rl1;
   d.l2pl1 = a.l2pl1;
   d.l2oct = a.l2oct;
+  d.nb0 = m->kind == 1 ? a.nb0 : nullptr;
   d.nb1 = a.nb1;
   d.nb2 = a.nb2;
   d.slots = a.slots;
   d.g0 = m->g0;
-  d.G.g0 = m->g0;
   d.G.N0 = a.N;
   d.G.cgx = a.cgx;
   d.G.l1gx = a.l1gx;
@@ -115,13 +115,14 @@ static int upload(jp_ctx* c) {
     for (long k = 0; k < chunks; ++k) L[EK_DENSE].push_back(make_int2(i, int(k)));
     // whole 2048-cell chunks only: a partial chunk would read and write past the member's lattice
     if (m->d.cells % kDense8Chunk) e.dense8_ok = false;
+    if (m->kind == 1) e.dense8_ok = false;  // AMR level 0 is tile-major: one cell per lane only
     for (long k = 0; k < m->d.cells / kDense8Chunk; ++k) L8.push_back(make_int2(i, int(k)));
     if (m->kind != 1) continue;
     const AmrState& a = m->amr;
     if (a.nf1) {
       for (int b = 0; b < a.nf1; ++b) L[EK_FILL1].push_back(make_int2(i, b));
       for (int b = 0; b < a.nf1; ++b) L[EK_STEP1].push_back(make_int2(i, b));
-      for (int b = 0; b < a.nf1; ++b) L[EK_STEP1R].push_back(make_int2(i, b));
+      for (int b = 0; b < a.nf1; ++b) L[EK_STEP1R].push_back(make_int2(i, b));  // slab units, one per tile
     }
     if (a.nf1 && a.nf2) {
       for (int b = 0; b < a.nf2; ++b) L[EK_FILL2].push_back(make_int2(i, b));
@@ -317,12 +318,13 @@ int ens_autotune(jp_ctx* c, int* out) {
 }
 
 // Sets (set = 1 or 8) or only reports (set = 0) the dense step's tile shape in cells per lane. 8 needs
-// every member's cell count to be a multiple of the 2048-cell chunk.
+// uniform members only (an AMR member's level 0 is tile-major), each a multiple of the 2048-cell chunk.
 int ens_dense_shape(jp_ctx* c, int set, int* current) {
   EnsState& e = *c->ens;
   if (int r = ens_check(c)) return r;
   if (set == 8) {
-    if (!e.dense8_ok) return fail(c, JP_ERR_ARG, "8 cells per lane needs every member's lattice to be a multiple of 2048 cells");
+    if (!e.dense8_ok)
+      return fail(c, JP_ERR_ARG, "8 cells per lane needs uniform members only, each lattice a multiple of 2048 cells");
     e.dense8 = true;
   } else if (set == 1) {
     e.dense8 = false;

@@ -22,12 +22,7 @@ __global__ __launch_bounds__(BS) void k_dense_init(typename S::raw* __restrict__
   using R = typename S::real;
   const uint32_t cell = blockIdx.x * BS + threadIdx.x;
   const uint32_t row = fdiv(cell, g.divN), x = cell - row * g.N, z = fdiv(row, g.divN), y = row - z * g.N;
-  const double X = (double(x) + 0.5) * kf + px, Y = (double(y) + 0.5) * kf + py, Z = (double(z) + 0.5) * kf + pz;
-  const double ux = u0 * sin(X) * cos(Y) * cos(Z);
-  const double uy = -u0 * cos(X) * sin(Y) * cos(Z);
-  const double p = (u0 * u0 / 16.0) * (cos(2 * X) + cos(2 * Y)) * (cos(2 * Z) + 2.0);
-  const double rho = 1.0 + 3.0 * p;
-  const EqPre<R> e = eq_pre<R>(R(rho), R(ux), R(uy), R(0));
+  const EqPre<R> e = tgv_equilibrium<R>(x, y, z, u0, kf, px, py, pz);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     f[g.at(q, z, y * g.N + x)] = S::template store<q>(feq<q>(e), enc);

@@ -71,11 +71,11 @@ def test_graph_replay_and_regrid_match():
 
 # members whose lattices split into whole 2048-cell chunks (edges that are multiples of 16): the dense step
 # can then run 8 cells per lane
+# 8 cells per lane takes uniform members whose lattices are multiples of 2048 cells
 MEMBERS16 = [
     {"n": 32, "re": 1600.0, "u0": 0.05, "amr": 0},
     {"n": 16, "re": 900.0, "u0": 0.04, "amr": 0, "phase": (0.3, 1.1, 2.0)},
-    {"n": 16, "re": 1600.0, "u0": 0.06, "amr": 1, "l1": jpabi.MODE_SENSOR, "l2": jpabi.MODE_SENSOR, "frac1": 0.4,
-     "frac2": 0.3, "adapt": 3, "phase": (1.0, 0.5, 0.25)},
+    {"n": 48, "re": 1600.0, "u0": 0.06, "amr": 0, "phase": (1.0, 0.5, 0.25)},
 ]
 
 
@@ -98,9 +98,11 @@ def test_dense_tile_shapes_match_single_contexts(prec):
                 assert np.array_equal(e.raw(lvl), a.raw(lvl)), (shape, m, lvl)
 
 
-def test_dense8_needs_whole_chunks():
-    """A member whose lattice is not a multiple of 2048 cells (24^3) keeps the dense step at 1 cell per lane."""
-    ens = jpabi.Ensemble(MEMBERS, maxgb=16.0)
+@pytest.mark.parametrize("members", [MEMBERS, MEMBERS16[:2] + [MEMBERS[2]]], ids=["ragged", "amr"])
+def test_dense8_needs_uniform_whole_chunks(members):
+    """A member whose lattice is not a multiple of 2048 cells (24^3), or an AMR member (whose level 0 is
+    tile-major), keeps the dense step at 1 cell per lane."""
+    ens = jpabi.Ensemble(members, maxgb=16.0)
     ens.init()
     with pytest.raises(jpabi.JPError) as e:
         ens.dense_shape(8)
