@@ -25,7 +25,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step(const ty
 // Last fine substep with the restriction into the parent level fused in (ptile: the parent tile of each
 // fine tile, l2pl1 or l1pcb); one slab of sibling tiles per unit (fine_step_restrict).
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) JP_RESTRICT_OCCUPANCY(S, kLoop) void k_fine_step_r(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                      const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                                      const int* __restrict__ dn, typename S::real om, float enc, float dec,
                                                      typename S::raw* __restrict__ cf, const int* __restrict__ ptile,
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
   });
 }
 template <class S, bool kLoop>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kLoop && sizeof(typename S::raw) == 2 ? 7 : (sizeof(typename S::raw) == 8 ? 4 : 8)))) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
+__global__ __launch_bounds__(512) JP_RESTRICT_OCCUPANCY(S, kLoop) void k_step_r_l2_fill_l1(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                                            const typename S::real* __restrict__ gh2,
                                                            const int* __restrict__ nb2, int n2, const int* __restrict__ dn2,
                                                            int w2, typename S::real om2, float enc2, float dec2,

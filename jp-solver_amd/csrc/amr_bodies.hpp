@@ -10,6 +10,10 @@ namespace jp {
 // (100-106) held them at 7. FP64 asks for 4, which its 84-106 vector registers already give (at 8 it
 // would spill).
 #define JP_FULL_OCCUPANCY(S) __attribute__((amdgpu_waves_per_eu(sizeof(typename S::raw) == 8 ? 4 : 8)))
+// The looping 16-bit forms of the restricting kernels ask for 7: within 64 registers the compiler spilled
+// (k_fine_step_r) or loaded the gathered populations one at a time (k_step_r_l2_fill_l1).
+#define JP_RESTRICT_OCCUPANCY(S, kLoop) \
+  __attribute__((amdgpu_waves_per_eu(kLoop && sizeof(typename S::raw) == 2 ? 7 : (sizeof(typename S::raw) == 8 ? 4 : 8))))
 
 __device__ __forceinline__ int pmod(int a, int n) {
   int r = a % n;
@@ -273,16 +277,21 @@ __device__ __forceinline__ void fine_step_restrict(const typename S::raw* __rest
   const int i = l & 7, j = l >> 3;
   const bool corner = !(i & 1) && !(j & 1);
   const int pp = (j >> 1) * 4 + (i >> 1);
-  R s[Q];
   if (!(w & 1)) {  // children c = 0..3 of plane 2pk
-    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(R(0), kept[decltype(qc)::value]); });
-    if (corner) sfor<Q>([&](auto qc) { L.part[sib][decltype(qc)::value][pp] = s[decltype(qc)::value]; });
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R s = add_children(R(0), kept[q]);
+      if (corner) L.part[sib][q][pp] = s;
+    });
   }
   __syncthreads();
   if (w & 1) {  // children c = 4..7 of plane 2pk + 1, continuing the running sum (every lane reads its parent's)
     const int pl = ((sib >> 1) * 4 + (j >> 1)) * 8 + (sib & 1) * 4 + (i >> 1);
-    sfor<Q>([&](auto qc) { s[decltype(qc)::value] = add_children(L.part[sib][decltype(qc)::value][pp], kept[decltype(qc)::value]); });
-    if (corner) sfor<Q>([&](auto qc) { L.avg[decltype(qc)::value][pl] = s[decltype(qc)::value]; });
+    sfor<Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const R s = add_children(L.part[sib][q][pp], kept[q]);
+      if (corner) L.avg[q][pl] = s;
+    });
   }
   __syncthreads();
   if (w == 0) {  // one full wave finishes the 64 parents of layer pz, lane = py * 8 + px
@@ -341,7 +350,7 @@ __device__ __forceinline__ float corner_weight(uint32_t e, int c) {
   return ((c & 1) ? tx : 1 - tx) * ((c & 2) ? ty : 1 - ty) * ((c & 4) ? tz : 1 - tz);
 }
 
-// Level 1 from the dense level 0 (prep0 over the coarse box; every box cell exists); kThreads lanes.
+// Level 1 from level 0 (prep0 over the coarse box; every box cell exists); kThreads lanes.
 template <class S, int kThreads = kFillThreads>
 __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                     const int* __restrict__ l1pcb, const int* __restrict__ l1oct,

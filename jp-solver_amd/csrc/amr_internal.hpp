@@ -68,7 +68,7 @@ hipError_t amr_fine_step_restrict(int prec, const void* fa, void* fb, const void
 hipError_t amr_l0_init(int prec, const AmrGeo& G, void* f, double u0, double kf, const double* phase, float enc,
                        hipStream_t st);
 hipError_t amr_l0_neighbours(int* nb0, int cgx, hipStream_t st);
-// Ghost records of level 1 (from the dense level 0 c0) and level 2 (from level 1 f1): prep values of
+// Ghost records of level 1 (from level 0, c0) and level 2 (from level 1 f1): prep values of
 // the parent box computed on the fly, trilinear interpolation of the slots whose source tile is absent.
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
                        const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, hipStream_t st);
