@@ -56,7 +56,20 @@ def cpu_baseline(target_s=15.0):
     glups = n ** 3 * steps / t / 1e9
     return {"value": glups, "unit": "GLUPS", "cores": oracle.lib().orc_threads(), "kind": "port",
             "sample": f"oracle uniform {n}^3 FP32 Re 1600, {steps} steps in {t:.1f} s "
-                      f"(OpenMP, {oracle.lib().orc_threads()} threads)"}
+                      f"(OpenMP, {oracle.lib().orc_threads()} threads)",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline_amr(target_s=8.0):

@@ -160,33 +160,86 @@ __device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], type
   });
 }
 
+// What a lane of a fine step needs to locate its pulled values: the neighbour entries (dz, dy, dx) =
+// (z ? ze : 0, y ? ye : 0, x ? xe : 0) as n<z><y><x>, where xe is -1 for i = 0, +1 for i = 7 and 0
+// otherwise (likewise ye, ze), and the in-tile source coordinates of the x and y shifts c = +1, 0, -1.
+struct PullLane {
+  int n000, n001, n010, n011, n100, n101, n110, n111;
+  int xe, ye, ze;
+  uint32_t sxp, sx0, sxm, syp, sy0, sym;
+};
+__device__ __forceinline__ PullLane pull_lane(const int* snb, int i, int j, int k) {
+  PullLane P;
+  P.xe = i == 0 ? -1 : (i == 7 ? 1 : 0);
+  P.ye = j == 0 ? -1 : (j == 7 ? 1 : 0);
+  P.ze = k == 0 ? -1 : (k == 7 ? 1 : 0);
+  const int x = P.xe + 1, y = (P.ye + 1) * 3, z = (P.ze + 1) * 9;
+  P.n000 = snb[9 + 3 + 1];
+  P.n001 = snb[9 + 3 + x];
+  P.n010 = snb[9 + y + 1];
+  P.n011 = snb[9 + y + x];
+  P.n100 = snb[z + 3 + 1];
+  P.n101 = snb[z + 3 + x];
+  P.n110 = snb[z + y + 1];
+  P.n111 = snb[z + y + x];
+  // the entries as register values: otherwise the compiler turns each direction's choice among them into a
+  // choice of address and keeps the struct in scratch memory
+  asm volatile("" : "+v"(P.n000), "+v"(P.n001), "+v"(P.n010), "+v"(P.n011));
+  asm volatile("" : "+v"(P.n100), "+v"(P.n101), "+v"(P.n110), "+v"(P.n111));
+  P.sxp = uint32_t((i - 1) & 7);
+  P.sx0 = uint32_t(i);
+  P.sxm = uint32_t((i + 1) & 7);
+  P.syp = uint32_t(((j - 1) & 7) * MB);
+  P.sy0 = uint32_t(j * MB);
+  P.sym = uint32_t(((j + 1) & 7) * MB);
+  return P;
+}
+// Source tile t of direction q for the lane (-1: the ghost record) and the element offset src of its
+// source cell within tile t. The lane crosses into the x-neighbour exactly when c_x = -xe (likewise y, z).
+template <int q>
+__device__ __forceinline__ void pull_source(PullLane P, int k, int& t, uint32_t& src) {
+  constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
+  const bool crx = cx != 0 && P.xe == -cx, cry = cy != 0 && P.ye == -cy, crz = cz != 0 && P.ze == -cz;
+  t = crx ? (cry ? P.n011 : P.n001) : (cry ? P.n010 : P.n000);
+  if constexpr (cz != 0)
+    if (crz) t = crx ? (cry ? P.n111 : P.n101) : (cry ? P.n110 : P.n100);
+  const uint32_t sx = cx > 0 ? P.sxp : (cx < 0 ? P.sxm : P.sx0), sy = cy > 0 ? P.syp : (cy < 0 ? P.sym : P.sy0);
+  src = uint32_t(q * MB3 + ((k - cz) & 7) * (MB * MB)) + sy + sx;
+}
+
 // Fine pull stream-collide, levels 1 and 2 (src/tgv3d_amr.cu:241-274, 331-356). nb holds the tile's
 // 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
 // to this level); missing sources come from the ghost record.
 // snb: the tile's neighbour entries, staged in LDS by the caller; g: the tile's ghost record (in HBM, or
-// in LDS where the caller has just computed it).
+// in LDS where the caller has just computed it). A wave must hold one z-plane of the tile (64 lanes of
+// one k), as every caller's lane = cell mapping gives.
+// The source of direction q lies in a neighbour tile only along the axes where the lane is on the tile's
+// edge facing -c_q: a lane at x-edge xe (-1 for i = 0, +1 for i = 7, else 0) crosses in x exactly when
+// c_x = -xe. So a lane needs at most the 8 entries (dz, dy, dx) in {0, ze} x {0, ye} x {0, xe}: it reads
+// them once, and each direction selects among the ones its non-zero components allow (no per-direction
+// LDS lookup and index arithmetic). Element offsets are 32-bit: a fine level holds at most kMaxFineTiles
+// tiles.
 template <class S, bool kKeep = false>
 __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
   using R = typename S::real;
   using raw = typename S::raw;
-  const int i = cell & 7, j = (cell >> 3) & 7, k = cell >> 6;
+  const int i = cell & 7, j = (cell >> 3) & 7;
+  const int k = __builtin_amdgcn_readfirstlane(cell >> 6);  // uniform: a wave is one z-plane of the tile
+  const PullLane P = pull_lane(snb, i, j, k);
   R fin[Q];
   if constexpr (sizeof(raw) == sizeof(R)) {
     // populations and ghost values share a type: the compiler selects the address, one load per direction
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
       if constexpr (q == 0) {
         fin[0] = S::template in<0>(fa[tidx(b, 0, cell)], dec);
       } else {
-        const int si = i - cx, sj = j - cy, sk = k - cz;
-        const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
-        const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
-        const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
-        const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        if (t >= 0) fin[q] = S::template in<q>(fa[tidx(t, q, ((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7))], dec);
+        int t;
+        uint32_t src;
+        pull_source<q>(P, k, t, src);
+        if (t >= 0) fin[q] = S::template in<q>(fa[__umul24(uint32_t(t), uint32_t(Q * MB3)) + src], dec);
         else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
       }
     });
@@ -197,26 +250,17 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
     raw v[Q];
     R gv[Q];
     uint32_t from_ghost = 0;
-    // a wave is one z-plane of the tile (lane = cell), so k and everything derived from it is scalar
-    const int kw = __builtin_amdgcn_readfirstlane(k);
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      constexpr int cx = CX(q), cy = CY(q), cz = CZ(q);
       gv[q] = R(0);
       if constexpr (q == 0) {
         v[0] = fa[tidx(b, 0, cell)];
       } else {
-        const int si = i - cx, sj = j - cy, sk = kw - cz;
-        const int dx = cx == 0 ? 0 : (si < 0 ? -1 : (si > 7 ? 1 : 0));
-        const int dy = cy == 0 ? 0 : (sj < 0 ? -1 : (sj > 7 ? 1 : 0));
-        const int dz = cz == 0 ? 0 : (sk < 0 ? -1 : (sk > 7 ? 1 : 0));
-        const int t = snb[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        // a ghost lane reads the same in-tile index of its own tile (in bounds, discarded); the direction's
-        // offset goes into the scalar base, so a lane adds only tile * 19 * 512 + cell (one 64-bit
-        // multiply-add and one shifted add)
-        const raw* fq = fa + q * MB3;
-        const uint32_t src = uint32_t(((sk & 7) * 8 + (sj & 7)) * 8 + (si & 7));
-        v[q] = fq[size_t(uint32_t(t >= 0 ? t : b)) * (Q * MB3) + src];
+        int t;
+        uint32_t src;
+        pull_source<q>(P, k, t, src);
+        // a ghost lane reads the same in-tile index of its own tile (in bounds, discarded)
+        v[q] = fa[__umul24(uint32_t(t >= 0 ? t : b), uint32_t(Q * MB3)) + src];
         if (t < 0) {
           gv[q] = g[ghost_slot(q, i, j, k)];
           from_ghost |= 1u << q;

@@ -117,9 +117,10 @@ int amr_sync_counts(jp_ctx* c) {
 }
 
 static bool pool_fits(const AmrState& a, long c1, long c2) {
-  // src/tgv3d_amr.cu:687-688: (f x2 + ghost box) per fine tile in FP32, +20 % for tables
+  // src/tgv3d_amr.cu:687-688: (f x2 + ghost box) per fine tile in FP32, +20 % for tables; and at most
+  // kMaxFineTiles tiles per level (32-bit element offsets in the fine steps)
   const double PB = double(Q) * 4 * (2 * MB3 + 1000);
-  return double(c1 + c2) * PB * 6 / 5 < a.budget;
+  return c1 <= kMaxFineTiles && c2 <= kMaxFineTiles && double(c1 + c2) * PB * 6 / 5 < a.budget;
 }
 
 static int alloc_l2_pool(jp_ctx* c) {
@@ -512,8 +513,8 @@ static int apply_resident(jp_ctx* c) {
   if (!a.resident_req || !amr_dynamic(c)) return JP_OK;
   const double per_tile = double(Q) * 4 * (2 * MB3 + 1000) * 6 / 5;  // pool_fits' bytes per fine tile
   const long room = std::max(0L, long(a.budget / per_tile) - 1);     // fine tiles of both levels
-  const long c1 = std::max<long>(a.cap1, std::min<long>(8L * a.cnb0, (room - a.cap2_alloc) / 8 * 8));
-  const long c2 = std::max<long>(a.cap2_alloc, std::min<long>(8L * c1, (room - c1) / 8 * 8));
+  const long c1 = std::max<long>(a.cap1, std::min<long>({8L * a.cnb0, (room - a.cap2_alloc) / 8 * 8, kMaxFineTiles}));
+  const long c2 = std::max<long>(a.cap2_alloc, std::min<long>({8L * c1, (room - c1) / 8 * 8, kMaxFineTiles}));
   const size_t e = prec_bytes(c->prec), r = real_bytes(c->prec), tile = size_t(Q) * MB3;
   if (c1 > a.cap1) {
     if (int rc = realloc_keep(c, &a.f1a, size_t(c1) * tile * e, size_t(a.nf1) * tile * e, "f1a")) return rc;

@@ -12,6 +12,9 @@ namespace jp {
 
 constexpr int kGhostSlots = 1824;
 constexpr int kNbr = 27;
+// Most tiles a fine level's pool holds: the fine steps address populations with 32-bit element offsets
+// (tile * 19 * 512 + offset < 2^32), a multiple of 8 (whole octets).
+constexpr long kMaxFineTiles = 441504;
 
 // Ghost fills interpolate from the 6^3 parent-level box starting at parent coordinate o/2 - 1 of a tile
 // at fine origin o. Fill-table entry of one ghost slot, source (si, sj, sk) in [-1, 8]^3 relative to the
