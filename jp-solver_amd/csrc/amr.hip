@@ -129,16 +129,22 @@ __global__ __launch_bounds__(512) JP_RESTRICT_OCCUPANCY(S, kLoop) void k_step_r_
                                                            const uint32_t* __restrict__ slots, int n1,
                                                            const int* __restrict__ dn1, float nqs01, float dec0) {
   using R = typename S::real;
-  __shared__ RestrictLds<R> RL;
-  __shared__ FillLds<R> L;
+  // the two parts run in different workgroups: their LDS overlaps
+  __shared__ union {
+    RestrictLds<R> rl;
+    FillLds<R> fl[2];
+  } U;
   if (int(blockIdx.x) < w2) {
     for_tiles_in<kLoop>(n2, dn2, blockIdx.x, w2, [=](int u, int lane) {
-      fine_step_restrict<S>(fa, fb, gh2, nb2, l2pl1, u, lane, om2, enc2, dec2, f1, nqi21, enc1, RL);
+      fine_step_restrict<S>(fa, fb, gh2, nb2, l2pl1, u, lane, om2, enc2, dec2, f1, nqi21, enc1, U.rl);
     });
     return;
   }
-  for_tiles_in<kLoop, false>(n1, dn1, int(blockIdx.x) - w2, int(gridDim.x) - w2, [=](int b, int lane) {
-    fill_l1_tile<S, 512>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs01, dec0, b, lane, L);
+  // the level-1 fill takes two tiles per workgroup (x-siblings 2p, 2p + 1, whose parent boxes overlap), a
+  // half of the lanes each: the fill is latency-bound, so twice the tiles in flight per workgroup
+  for_tiles_in<kLoop, false, 2>(n1, dn1, int(blockIdx.x) - w2, int(gridDim.x) - w2, [=](int p, int lane) {
+    const int h = lane >> 8;
+    fill_l1_tile<S, 256>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs01, dec0, 2 * p + h, lane & 255, U.fl[h]);
   });
 }
 
@@ -428,7 +434,8 @@ hipError_t amr_step_r_l2_fill_l1(int prec, const AmrGeo& G, const void* f2a, voi
                                  Tiles t2, double om2, float enc2, float dec2, void* f1, const int* l2pl1,
                                  float nqi21, float enc1, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
                                  const int* nb1, const uint32_t* slots, Tiles t1, float nqs01, float dec0, hipStream_t st) {
-  const int w2 = part8(t2), w1 = part8(t1);
+  // the level-1 fill's workgroups take tile pairs
+  const int w2 = part8(t2), w1 = part8({t1.count, (t1.width + 1) / 2});
   if (w1 + w2 == 0) return hipSuccess;
   JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_step_r_l2_fill_l1<S, kLoop>), dim3(w2 + w1), dim3(512), 0, st,
                                                     RAW(f2a), WRAW(f2b), REAL(gh2), nb2, t2.width, t2.count, w2,

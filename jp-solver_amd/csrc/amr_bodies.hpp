@@ -62,13 +62,14 @@ __device__ __forceinline__ int opaque_lane() {
 // for_tiles_in: the same over a part of a launch, workgroups wg in [0, width) of it (width a multiple of 8
 // and the part starting at a multiple of 8, so that wg keeps its XCD); the one-tile form then skips tiles
 // from n up (width is n rounded up to a multiple of 8).
-template <bool kLoop, bool kSync = true, class F>
+// per > 1: the unit is a group of per consecutive tiles, n and *dn still count tiles (a multiple of per).
+template <bool kLoop, bool kSync = true, int per = 1, class F>
 __device__ __forceinline__ void for_tiles_in(int n, const int* __restrict__ dn, int wg, int width, F&& body) {
   if constexpr (!kLoop) {
     const int b = xcd_tile(wg, width);
-    if (b < n) body(b, int(threadIdx.x));
+    if (b < n / per) body(b, int(threadIdx.x));
   } else {
-    const int nd = *dn;
+    const int nd = *dn / per;
     for (int v = wg; v < nd; v += width) {
       if constexpr (kSync)
         if (v != wg) __syncthreads();
