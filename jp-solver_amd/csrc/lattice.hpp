@@ -185,26 +185,16 @@ struct FormF64 {
   template <int q> __device__ static inline raw out(real f, float) { return f; }
   template <int q> __device__ static inline real from_full(real f) { return f; }
 };
-// -0.0f as a register value. fma(x, y, -0) rounds x*y once, exactly as the product does (x*y + (-0) = x*y
-// for every x*y, zeros included); the compiler folds a literal -0 addend back into a multiply, but with the
-// addend in a register a binary16 x becomes one v_fma_mix_f32 (conversion and multiply in one instruction).
-__device__ inline float neg_zero() {
-  float z = -0.0f;
-  asm("" : "+v"(z));
-  return z;
-}
 struct FormF16 {
   using raw = _Float16;
   using real = float;
   static constexpr int id = 1;
   static constexpr bool kShift = true;
-  template <int q> __device__ static inline real load(raw r, float dec) {
-    return __builtin_fmaf(float(r), dec, neg_zero()) + W<float>(q);
-  }
+  template <int q> __device__ static inline real load(raw r, float dec) { return float(r) * dec + W<float>(q); }
   template <int q> __device__ static inline raw store(real f, float enc) {
     return static_cast<_Float16>((f - W<float>(q)) * enc);
   }
-  template <int q> __device__ static inline real in(raw r, float dec) { return __builtin_fmaf(float(r), dec, neg_zero()); }
+  template <int q> __device__ static inline real in(raw r, float dec) { return float(r) * dec; }
   template <int q> __device__ static inline raw out(real d, float enc) { return static_cast<_Float16>(d * enc); }
   template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
 };
