@@ -223,9 +223,8 @@ int jp_set_variant(jp_ctx* c, int variant);
 /* Two fused time steps per launch for the uniform lattice (not in the reference, which launches one
  * k_step per step, src/tgv3d_uniform.cu:166-167; a tuning knob): a workgroup streams a column of the
  * lattice and keeps the intermediate step in LDS, so HBM sees one read and one write per two steps plus
- * the column halos. On by default for FP32 storage where a column shape divides N (a multiple of 32);
- * the 16-bit forms fuse only when a shape is picked (jp_set_step_pair_shape) or jp_autotune keeps one;
- * FP64 and other edges run single steps. Results are bit-identical either way; an odd step count ends
+ * the column halos. On by default for FP32, FP16 and BF16 storage where a column shape divides N (a
+ * multiple of 32); FP64 and other edges run single steps. Results are bit-identical either way; an odd step count ends
  * with a single step. Resets the column shape to the first in preference order that divides N. */
 int jp_set_step_pairs(jp_ctx* c, int on);
 /* Selects the fused pair's column shape by its index in the library's preference order (0: 128 x 4,

@@ -41,20 +41,9 @@ template <class S, bool kLoop>
 __global__ __launch_bounds__(256) JP_FULL_OCCUPANCY(S) void k_fill_l1(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                                  const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                                  const int* __restrict__ nb1, const uint32_t* __restrict__ slots,
-                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec) {
+                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec, float enc1) {
   __shared__ FillLds<typename S::real> L;
-  for_tiles<kLoop, false>(dn, [=](int b, int lane) { fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, b, lane, L); });
-}
-template <class S, bool kLoop>
-__global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
-                                                 const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
-                                                 const int* __restrict__ f1pos, const int* __restrict__ l1grid,
-                                                 const int* __restrict__ nb2, const uint32_t* __restrict__ slots,
-                                                 const int* __restrict__ dn, AmrGeo G, float nqs, float dec) {
-  __shared__ FillLds<typename S::real> L;
-  for_tiles<kLoop, false>(dn, [=](int b, int lane) {
-    fill_l2_tile<S>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec, b, lane, L);
-  });
+  for_tiles<kLoop, false>(dn, [=](int b, int lane) { fill_l1_tile<S>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs, dec, enc1, b, lane, L); });
 }
 // Level-2 ghost fill fused with the first level-2 substep of the pair that reads it: the workgroup fills
 // its tile's ghost record (written to gh2 for the pair's second substep, and kept in LDS) and steps the tile
@@ -62,7 +51,7 @@ __global__ __launch_bounds__(256) void k_fill_l2(const typename S::raw* __restri
 template <class S, bool kLoop>
 __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fill_step_l2(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                                       const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
-                                                      const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                                      const int* __restrict__ f1pos, const int* __restrict__ nb1,
                                                       const int* __restrict__ nb2, const uint32_t* __restrict__ slots,
                                                       const int* __restrict__ dn, AmrGeo G, float nqs, float dec1,
                                                       const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
@@ -71,7 +60,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fill_step_l2(const
   __shared__ FillLds<R> L;
   __shared__ R ghost[kGhostSlots];
   for_tiles<kLoop, false>(dn, [=](int b, int lane) {
-    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec1, b, lane, L, ghost);
+    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, nb1, nb2, slots, G, nqs, dec1, enc2, b, lane, L, ghost);
     __syncthreads();
     fine_step_cell<S>(fa, fb, ghost, L.nb, om, enc2, dec2, b, lane);
   });
@@ -93,7 +82,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
                                                             const typename S::raw* __restrict__ f1,
                                                             typename S::real* __restrict__ gh2, const int* __restrict__ l2pl1,
                                                             const int* __restrict__ l2oct, const int* __restrict__ f1pos,
-                                                            const int* __restrict__ l1grid, const int* __restrict__ nb2,
+                                                            const int* __restrict__ nb1, const int* __restrict__ nb2,
                                                             const uint32_t* __restrict__ slots, int n2,
                                                             const int* __restrict__ dn2, AmrGeo G, float nqs, float dec1,
                                                             const typename S::raw* __restrict__ fa,
@@ -111,7 +100,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
     return;
   }
   for_tiles_in<kLoop, false>(n2, dn2, int(blockIdx.x) - nd, int(gridDim.x) - nd, [=](int b, int lane) {
-    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, l1grid, nb2, slots, G, nqs, dec1, b, lane, L, ghost);
+    fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, nb1, nb2, slots, G, nqs, dec1, enc2, b, lane, L, ghost);
     __syncthreads();
     fine_step_cell<S>(fa, fb, ghost, L.nb, om2, enc2, dec2, b, lane);
   });
@@ -144,7 +133,7 @@ __global__ __launch_bounds__(512) JP_RESTRICT_OCCUPANCY(S, kLoop) void k_step_r_
   // half of the lanes each: the fill is latency-bound, so twice the tiles in flight per workgroup
   for_tiles_in<kLoop, false, 2>(n1, dn1, int(blockIdx.x) - w2, int(gridDim.x) - w2, [=](int p, int lane) {
     const int h = lane >> 8;
-    fill_l1_tile<S, 256>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs01, dec0, 2 * p + h, lane & 255, U.fl[h]);
+    fill_l1_tile<S, 256>(c0, gh1, l1pcb, l1oct, nb1, slots, G, nqs01, dec0, enc1, 2 * p + h, lane & 255, U.fl[h]);
   });
 }
 
@@ -388,28 +377,19 @@ hipError_t amr_fine_step_restrict(int prec, const void* fa, void* fb, const void
 }
 
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, hipStream_t st) {
+                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, float enc1, hipStream_t st) {
   if (t.width <= 0) return hipSuccess;
   JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_l1<S, kLoop>), dim3(t.width), dim3(256), 0, st, RAW(c0), WREAL(gh1), l1pcb, l1oct,
-                                       nb1, slots, t.count, G, nqs, dec)));
-  return hipGetLastError();
-}
-
-hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
-                       float dec, hipStream_t st) {
-  if (t.width <= 0) return hipSuccess;
-  JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_l2<S, kLoop>), dim3(t.width), dim3(256), 0, st, RAW(f1), WREAL(gh2), l2pl1, l2oct,
-                                       f1pos, l1grid, nb2, slots, t.count, G, nqs, dec)));
+                                       nb1, slots, t.count, G, nqs, dec, enc1)));
   return hipGetLastError();
 }
 
 hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                            const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
+                            const int* f1pos, const int* nb1, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
                             float dec1, const void* fa, void* fb, double omega, float enc2, float dec2, hipStream_t st) {
   if (t.width <= 0) return hipSuccess;
   JP_DISPATCH(prec, JP_TILES(t, hipLaunchKernelGGL((k_fill_step_l2<S, kLoop>), dim3(t.width), dim3(512), 0, st, RAW(f1),
-                                                   WREAL(gh2), l2pl1, l2oct, f1pos, l1grid, nb2, slots, t.count, G, nqs, dec1,
+                                                   WREAL(gh2), l2pl1, l2oct, f1pos, nb1, nb2, slots, t.count, G, nqs, dec1,
                                                    RAW(fa), WRAW(fb), typename S::real(omega), enc2, dec2)));
   return hipGetLastError();
 }
@@ -419,13 +399,13 @@ static inline int part8(Tiles t) { return t.width > 0 ? (t.width + 7) / 8 * 8 : 
 
 hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, const int* nb0, double om0,
                                   float enc0, float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                                  const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t2,
+                                  const int* f1pos, const int* nb1, const int* nb2, const uint32_t* slots, Tiles t2,
                                   float nqs12, float dec1, const void* f2a, void* f2b, double om2, float enc2, float dec2,
                                   hipStream_t st) {
   const int n0 = G.cgx * G.cgx * G.cgx, nd = (n0 + 7) / 8 * 8, w2 = part8(t2);
   JP_DISPATCH(prec, JP_TILES(t2, hipLaunchKernelGGL((k_dense_fill_step_l2<S, kLoop>), dim3(nd + w2), dim3(512), 0, st,
                                                     RAW(c0a), WRAW(c0b), nb0, n0, typename S::real(om0), enc0, dec0, nd,
-                                                    RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos, l1grid, nb2, slots, t2.width,
+                                                    RAW(f1), WREAL(gh2), l2pl1, l2oct, f1pos, nb1, nb2, slots, t2.width,
                                                     t2.count, G, nqs12, dec1, RAW(f2a), WRAW(f2b),
                                                     typename S::real(om2), enc2, dec2)));
   return hipGetLastError();

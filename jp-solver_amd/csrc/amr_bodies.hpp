@@ -101,7 +101,7 @@ template <class S> __device__ __forceinline__ void load_l0(const typename S::raw
 template <class S, bool kKeep = false>
 __device__ __forceinline__ void collide_store(typename S::real (&fin)[Q], typename S::real om, typename S::raw* __restrict__ fb,
                                      int b, int cell, float enc, typename S::real (&kept)[Q], float dec) {
-  const Collider<S> col(fin);
+  const Collider<S> col(fin, dec, enc);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], om), enc);
@@ -212,7 +212,7 @@ __device__ __forceinline__ void pull_source(PullLane P, int k, int& t, uint32_t&
 // 27 same-level neighbours ((dz+1)*9 + (dy+1)*3 + dx+1, -1 where the neighbour position is not refined
 // to this level); missing sources come from the ghost record.
 // snb: the tile's neighbour entries, staged in LDS by the caller; g: the tile's ghost record (in HBM, or
-// in LDS where the caller has just computed it). A wave must hold one z-plane of the tile (64 lanes of
+// in LDS where the caller has just computed it), whose values are in the form's working representation. A wave must hold one z-plane of the tile (64 lanes of
 // one k), as every caller's lane = cell mapping gives.
 // The source of direction q lies in a neighbour tile only along the axes where the lane is on the tile's
 // edge facing -c_q: a lane at x-edge xe (-1 for i = 0, +1 for i = 7, else 0) crosses in x exactly when
@@ -241,7 +241,7 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
         uint32_t src;
         pull_source<q>(P, k, t, src);
         if (t >= 0) fin[q] = S::template in<q>(fa[__umul24(uint32_t(t), uint32_t(Q * MB3)) + src], dec);
-        else fin[q] = S::template from_full<q>(g[ghost_slot(q, i, j, k)]);
+        else fin[q] = g[ghost_slot(q, i, j, k)];
       }
     });
   } else {
@@ -274,7 +274,7 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
       // into the branch above, where it would wait for the load
       R gq = gv[q];
       asm volatile("" : "+v"(gq));
-      fin[q] = (from_ghost >> q) & 1u ? S::template from_full<q>(gq) : S::template in<q>(v[q], dec);
+      fin[q] = (from_ghost >> q) & 1u ? gq : S::template in<q>(v[q], dec);
     });
   }
   collide_store<S, kKeep>(fin, om, fb, b, cell, enc, kept, dec);
@@ -369,12 +369,15 @@ template <class R> __device__ __forceinline__ void prep_values(R* v, float nqs) 
 // prep values computed in place (feq + nqs * regularize(f - feq) of the parent populations, exactly
 // what the reference's prep pass stores for these cells), then interpolates each slot from LDS. A slot
 // whose source lies in a same-level neighbour tile (nb >= 0) is not a ghost and is skipped, and only the
-// box cells that some remaining slot reads are staged (a tile with all 26 neighbours stages nothing).
+// box cells that some remaining slot reads are staged (a tile with all 26 neighbours stages nothing). The
+// record holds each interpolated population in the working representation of the fine level (enc: its
+// scale), so the fine steps read ghost values as they read decoded populations.
 constexpr int kBox = kFillBox, kBox3 = kBox * kBox * kBox;
 constexpr int kFillThreads = 256;
 template <class R> struct FillLds {
   R box[Q * kBox3];
   int nb[kNbr];
+  int nbp[kNbr];  // level 2: the 27 level-1 neighbours of the parent tile
   int ok[kBox3];  // level 2: the parent cell exists (level-1 tiles are scattered)
 };
 // Whether box cell (li, lj, lk) is a trilinear corner of some ghost slot. Slots toward a neighbour
@@ -400,7 +403,7 @@ template <class S, int kThreads = kFillThreads>
 __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__ c0, typename S::real* __restrict__ gh1,
                                     const int* __restrict__ l1pcb, const int* __restrict__ l1oct,
                                     const int* __restrict__ nb1, const uint32_t* __restrict__ slots, const AmrGeo& G,
-                                    float nqs, float dec, int fb, int tid, FillLds<typename S::real>& L) {
+                                    float nqs, float dec, float enc, int fb, int tid, FillLds<typename S::real>& L) {
   using R = typename S::real;
   constexpr int kSlotsPerLane = (kGhostSlots + kThreads - 1) / kThreads;
   const int cb = l1pcb[fb], oct = l1oct[fb];
@@ -434,19 +437,22 @@ __device__ __forceinline__ void fill_l1_tile(const typename S::raw* __restrict__
     R out = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) out += R(corner_weight(e, c)) * bq[((c >> 2) * kBox + ((c >> 1) & 1)) * kBox + (c & 1)];
-    gh1[size_t(fb) * kGhostSlots + tid + k * kThreads] = out;
+    gh1[size_t(fb) * kGhostSlots + tid + k * kThreads] = ghost_value<S>(out, e & 31, enc);
   }
 }
 
 // Level 2 from the scattered level 1: box cells without a level-1 tile are dropped and the remaining
 // weights renormalised (src/tgv3d_amr.cu:133-144, 187-202). kThreads lanes; lg (optional): an LDS copy
-// of the tile's ghost record for a step fused behind the fill.
+// of the tile's ghost record for a step fused behind the fill. The box lies within the parent level-1
+// tile and its 26 neighbours, so a box cell's level-1 tile is read from the parent's neighbour entries
+// (nb1, staged with the tile's own) rather than looked up in the level-1 grid per cell, which put one
+// more dependent load before the box loads.
 template <class S, int kThreads = kFillThreads>
 __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__ f1, typename S::real* __restrict__ gh2,
                                     const int* __restrict__ l2pl1, const int* __restrict__ l2oct,
-                                    const int* __restrict__ f1pos, const int* __restrict__ l1grid,
+                                    const int* __restrict__ f1pos, const int* __restrict__ nb1,
                                     const int* __restrict__ nb2, const uint32_t* __restrict__ slots, const AmrGeo& G,
-                                    float nqs, float dec, int b, int tid, FillLds<typename S::real>& L,
+                                    float nqs, float dec, float enc, int b, int tid, FillLds<typename S::real>& L,
                                     typename S::real* lg = nullptr) {
   using R = typename S::real;
   constexpr int kSlotsPerLane = (kGhostSlots + kThreads - 1) / kThreads;
@@ -462,13 +468,15 @@ __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__
   }
   __syncthreads();  // a previous tile of this workgroup is done with the LDS
   if (tid < kNbr) L.nb[tid] = nb2[size_t(b) * kNbr + tid];
+  else if (tid < 2 * kNbr) L.nbp[tid - kNbr] = nb1[size_t(pl1) * kNbr + tid - kNbr];
   __syncthreads();
   for (int c = tid; c < kBox3; c += kThreads) {
     const int li = c % kBox, lj = (c / kBox) % kBox, lk = c / (kBox * kBox);
     L.ok[c] = 0;
     if (!box_cell_needed(L.nb, li, lj, lk)) continue;
-    const int ci = pmod(bx0 + li, G.N1), cj = pmod(by0 + lj, G.N1), ck = pmod(bz0 + lk, G.N1);
-    const int lb = l1grid[((ck >> 3) * G.l1gx + (cj >> 3)) * G.l1gx + (ci >> 3)];
+    // level-1 coordinates, one cell outside the grid at most (-1 or N1: the periodic neighbour tile)
+    const int ci = bx0 + li, cj = by0 + lj, ck = bz0 + lk;
+    const int lb = L.nbp[((ck >> 3) - pz + 1) * 9 + ((cj >> 3) - py + 1) * 3 + (ci >> 3) - px + 1];
     if (lb < 0) continue;
     L.ok[c] = 1;
     R v[Q];
@@ -495,8 +503,9 @@ __device__ __forceinline__ void fill_l2_tile(const typename S::raw* __restrict__
       out += R(wgt) * bq[o];
     }
     if (wsum > 0.f && wsum < 0.999f) out *= R(1) / R(wsum);
-    gh2[size_t(b) * kGhostSlots + tid + k * kThreads] = out;
-    if (lg) lg[tid + k * kThreads] = out;
+    const R w = ghost_value<S>(out, e & 31, enc);
+    gh2[size_t(b) * kGhostSlots + tid + k * kThreads] = w;
+    if (lg) lg[tid + k * kThreads] = w;
   }
 }
 

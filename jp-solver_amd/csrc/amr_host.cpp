@@ -559,7 +559,7 @@ static int level2_pair(jp_ctx* c, const AmrGeo& G, hipStream_t st) {
   if (a.skipprep)
     HCK(c, amr_fine_step(p, 2, a.f2a, a.f2b, a.gh2, a.nb2, t2, c->d.omega2, enc(c, 2), dec(c, 2), st));
   else  // fill and first substep in one launch
-    HCK(c, amr_fill_step_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12),
+    HCK(c, amr_fill_step_l2(p, G, a.f1a, a.gh2, a.l2pl1, a.l2oct, a.f1pos, a.nb1, a.nb2, a.slots, t2, float(c->d.nqs12),
                             dec(c, 1), a.f2a, a.f2b, c->d.omega2, enc(c, 2), dec(c, 2), st));
   std::swap(a.f2a, a.f2b);
   if (a.skiprestr)
@@ -588,7 +588,7 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
   const Tiles t1 = tiles(c, 1), t2 = tiles(c, 2);
   if (l2 && !a.skipprep && !a.skiprestr) {
     HCK(c, amr_dense_fill_step_l2(p, G, c->c0a, c->c0b, a.nb0, c->d.omega0, enc(c, 0), dec(c, 0), a.f1a, a.gh2, a.l2pl1, a.l2oct,
-                                  a.f1pos, a.l1grid, a.nb2, a.slots, t2, float(c->d.nqs12), dec(c, 1), a.f2a, a.f2b,
+                                  a.f1pos, a.nb1, a.nb2, a.slots, t2, float(c->d.nqs12), dec(c, 1), a.f2a, a.f2b,
                                   c->d.omega2, enc(c, 2), dec(c, 2), st));
     std::swap(c->c0a, c->c0b);
     std::swap(a.f2a, a.f2b);
@@ -609,7 +609,7 @@ int amr_base_step(jp_ctx* c, hipStream_t st) {
     if (int r = l0_step(c, st)) return r;
     if (!a.resident && !a.nf1) return JP_OK;
     if (!a.skipprep)
-      HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), st));
+      HCK(c, amr_fill_l1(p, G, c->c0a, a.gh1, a.l1pcb, a.l1oct, a.nb1, a.slots, t1, float(c->d.nqs01), dec(c, 0), enc(c, 1), st));
     if (l2) HCK(c, hipStreamWaitEvent(st, ev_join, 0));
   }
   HCK(c, amr_fine_step(p, 1, a.f1a, a.f1b, a.gh1, a.nb1, t1, c->d.omega1, enc(c, 1), dec(c, 1), st));

@@ -71,17 +71,15 @@ hipError_t amr_fine_step_restrict(int prec, const void* fa, void* fb, const void
 hipError_t amr_l0_init(int prec, const AmrGeo& G, void* f, double u0, double kf, const double* phase, float enc,
                        hipStream_t st);
 hipError_t amr_l0_neighbours(int* nb0, int cgx, hipStream_t st);
-// Ghost records of level 1 (from level 0, c0) and level 2 (from level 1 f1): prep values of
-// the parent box computed on the fly, trilinear interpolation of the slots whose source tile is absent.
+// Ghost records of level 1 (from level 0, c0): prep values of the parent box computed on the fly,
+// trilinear interpolation of the slots whose source tile is absent, stored in the working representation
+// of the form at level 1's scale enc1.
 hipError_t amr_fill_l1(int prec, const AmrGeo& G, const void* c0, void* gh1, const int* l1pcb, const int* l1oct,
-                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, hipStream_t st);
-hipError_t amr_fill_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                       const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
-                       float dec, hipStream_t st);
+                       const int* nb1, const uint32_t* slots, Tiles t, float nqs, float dec, float enc1, hipStream_t st);
 // The level-2 fill fused with the first level-2 substep (fa -> fb): the ghost record is written to gh2 for
 // the second substep and read from LDS by this one.
 hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                            const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
+                            const int* f1pos, const int* nb1, const int* nb2, const uint32_t* slots, Tiles t, float nqs,
                             float dec1, const void* fa, void* fb, double omega, float enc2, float dec2, hipStream_t st);
 // The first two launches of a coarse step with level 2 present, each carrying two independent parts: the
 // level-0 step (c0a -> c0b) with the fused level-2 fill and first substep (f2a -> f2b), then the second
@@ -89,7 +87,7 @@ hipError_t amr_fill_step_l2(int prec, const AmrGeo& G, const void* f1, void* gh2
 // from the new level 0 (c0).
 hipError_t amr_dense_fill_step_l2(int prec, const AmrGeo& G, const void* c0a, void* c0b, const int* nb0, double om0,
                                   float enc0, float dec0, const void* f1, void* gh2, const int* l2pl1, const int* l2oct,
-                                  const int* f1pos, const int* l1grid, const int* nb2, const uint32_t* slots, Tiles t2,
+                                  const int* f1pos, const int* nb1, const int* nb2, const uint32_t* slots, Tiles t2,
                                   float nqs12, float dec1, const void* f2a, void* f2b, double om2, float enc2, float dec2,
                                   hipStream_t st);
 hipError_t amr_step_r_l2_fill_l1(int prec, const AmrGeo& G, const void* f2a, void* f2b, const void* gh2, const int* nb2,

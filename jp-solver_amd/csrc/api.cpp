@@ -52,11 +52,12 @@ void ctx_free(jp_ctx* c, void* p) {
 // 58.2-61.9 GLUPS in direction planes, single steps even), FP16 (73.1-73.5 against 70.3-72.0) and BF16
 // (68.7 against 67.9); direction planes for FP64 (17.9-18.3 against 15.3-16.0).
 bool default_layers(int prec) { return prec != JP_PREC_FP64; }
-// Elements after each direction row of a z-layer. 32 where the lattice runs fused pairs (FP32 with a
-// column shape that divides N): a sweep of the pad at 512^3 (128 x 4 / 64 x 8 columns, GLUPS) gave 0:
+// Elements after each direction row of a z-layer. 32 where the lattice runs fused pairs (FP32, FP16 or
+// BF16 with a column shape that divides N): a sweep of the FP32 pad at 512^3 (128 x 4 / 64 x 8, GLUPS) gave 0:
 // 58.8 / 51.4, 16: 60.5 / 59.7, 32: 64.6 / 64.2, 48: 61.4 / 60.8, 64-160: 63.5-64.2, 256: 62.4 / 52.0,
-// 512-2048: 56.6-61.5 / 49.5-51.5, while the single steps ran faster without a pad (FP32 34.9 against
-// 34.3, FP16 80.2 against 75.4 in the same process), so every other lattice keeps 0.
+// 512-2048: 56.6-61.5 / 49.5-51.5 (the FP16 pair: 83.0 against 82.7 without a pad), while the single
+// steps ran faster without a pad (FP32 34.9 against 34.3, FP16 80.2 against 75.4 in the same process), so
+// every other lattice keeps 0.
 uint64_t default_layer_pad(int prec, int N) {
   int shape[4];
   return step_pair_shape(prec, N, -1, shape) ? 32 : 0;

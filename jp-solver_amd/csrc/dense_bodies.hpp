@@ -92,7 +92,7 @@ __device__ inline void dense_step_vec(const typename S::raw* __restrict__ fa, ty
   for (int j = 0; j < V; ++j) {
     R fin[Q];
     sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = f[decltype(qc)::value][j]; });
-    const Collider<S> col(fin);
+    const Collider<S> col(fin, dec, enc);
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       f[q][j] = col.template post<q>(fin[q], omega);
@@ -125,7 +125,7 @@ __device__ inline void dense_step_cell(const typename S::raw* __restrict__ fa, t
     if constexpr (kNtLoad) fin[q] = S::template in<q>(__builtin_nontemporal_load(p), dec);
     else fin[q] = S::template in<q>(*p, dec);
   });
-  const Collider<S> col(fin);
+  const Collider<S> col(fin, dec, enc);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const typename S::raw v = S::template out<q>(col.template post<q>(fin[q], omega), enc);

@@ -62,11 +62,11 @@ __global__ __launch_bounds__(256) JP_FULL_OCCUPANCY(S) void kb_fill(const Member
     const MemberDesc& d = md[w.x];
     const int lane = opaque_lane();
     if constexpr (kLevel == 1)
-      fill_l1_tile<S>(cr<S>(d.c0[p]), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.nb1, d.slots, d.G, d.nqs01, d.dec[0], w.y,
+      fill_l1_tile<S>(cr<S>(d.c0[p]), static_cast<R*>(d.gh1), d.l1pcb, d.l1oct, d.nb1, d.slots, d.G, d.nqs01, d.dec[0], d.enc[1], w.y,
                       lane, L);
     else
-      fill_l2_tile<S>(cr<S>(d.f1[p]), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.l1grid, d.nb2, d.slots, d.G,
-                      d.nqs12, d.dec[1], w.y, lane, L);
+      fill_l2_tile<S>(cr<S>(d.f1[p]), static_cast<R*>(d.gh2), d.l2pl1, d.l2oct, d.f1pos, d.nb1, d.nb2, d.slots, d.G,
+                      d.nqs12, d.dec[1], d.enc[2], w.y, lane, L);
   }
 }
 // items: (member, fine tile)

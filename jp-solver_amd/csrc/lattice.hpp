@@ -161,8 +161,9 @@ template <class R> __device__ inline void regularize(R* n) {
 // A form maps the stored element of direction q to the working type and back: load / store convert
 // populations f. FP16/BF16 hold the shifted value (f - w_q) * enc with enc a power of two (dec = 1/enc
 // exactly), and collide in that shifted representation (kShift): in / out convert the collision's
-// working value (f for the plain forms, d = f - w_q for the shifted ones), from_full maps a population
-// f (ghost values) to it.
+// working value. The working value is f for the plain forms and the stored deviation itself,
+// D = (f - w_q) * enc, for the shifted ones: the collision takes the scale into its constants (Collider),
+// so neither the decode nor the encode multiplies. Ghost records hold working values too (ghost_value).
 struct FormF32 {
   using raw = float;
   using real = float;
@@ -172,7 +173,6 @@ struct FormF32 {
   template <int q> __device__ static inline raw store(real f, float) { return f; }
   template <int q> __device__ static inline real in(raw r, float) { return r; }
   template <int q> __device__ static inline raw out(real f, float) { return f; }
-  template <int q> __device__ static inline real from_full(real f) { return f; }
 };
 struct FormF64 {
   using raw = double;
@@ -183,7 +183,6 @@ struct FormF64 {
   template <int q> __device__ static inline raw store(real f, float) { return f; }
   template <int q> __device__ static inline real in(raw r, float) { return r; }
   template <int q> __device__ static inline raw out(real f, float) { return f; }
-  template <int q> __device__ static inline real from_full(real f) { return f; }
 };
 struct FormF16 {
   using raw = _Float16;
@@ -194,9 +193,13 @@ struct FormF16 {
   template <int q> __device__ static inline raw store(real f, float enc) {
     return static_cast<_Float16>((f - W<float>(q)) * enc);
   }
-  template <int q> __device__ static inline real in(raw r, float dec) { return float(r) * dec; }
-  template <int q> __device__ static inline raw out(real d, float enc) { return static_cast<_Float16>(d * enc); }
-  template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
+  template <int q> __device__ static inline real in(raw r, float) { return float(r); }
+  template <int q> __device__ static inline raw out(real d, float) {
+    // the empty asm keeps the conversion a separate rounding: fused into the multiply-add that produced d
+    // (v_fma_mixlo_f16) the value would be rounded once, from the exact result, instead of twice
+    asm volatile("" : "+v"(d));
+    return static_cast<_Float16>(d);
+  }
 };
 // binary32 -> bfloat16, round to nearest even: one v_cvt_pk_bf16_f32 on gfx950 (the integer form, add
 // 0x7fff plus the kept lsb and truncate, took five instructions per value); NaN stays NaN.
@@ -210,9 +213,8 @@ struct FormBF16 {
     return __uint_as_float(uint32_t(r) << 16) * dec + W<float>(q);
   }
   template <int q> __device__ static inline raw store(real f, float enc) { return bf16_rne((f - W<float>(q)) * enc); }
-  template <int q> __device__ static inline real in(raw r, float dec) { return __uint_as_float(uint32_t(r) << 16) * dec; }
-  template <int q> __device__ static inline raw out(real d, float enc) { return bf16_rne(d * enc); }
-  template <int q> __device__ static inline real from_full(real f) { return f - W<float>(q); }
+  template <int q> __device__ static inline real in(raw r, float) { return __uint_as_float(uint32_t(r) << 16); }
+  template <int q> __device__ static inline raw out(real d, float) { return bf16_rne(d); }
 };
 
 // Fused multiply-add in the working type (one rounding, IEEE fusedMultiplyAdd).
@@ -232,31 +234,46 @@ template <class R> __device__ __forceinline__ R fmx(R a, R b, R c) {
 // rho - 1), d_q) with g_q = fma(cu/2, cu, -t). In exact arithmetic this is the same collision; in FP32
 // every operand is then of the order of the deviations, whereas summing w_q + d_q rounds rho at the fixed
 // low bits of the w_q, a bias of ~1e-7 per cell and step that would show as a linear drift of the mass.
+// They work on the stored scale, D_q = enc d_q: the sums of D_q times dec give rho - 1 and rho u, and the
+// relaxation reads fma(-om, fma(-(w_q enc), fma(rho, g_q + cu, rho - 1), D_q), D_q). Multiplying by a power
+// of two is exact and commutes with rounding while values stay in the normal range (stored deviations are
+// at least 2^-24 dec and, like w_q enc, far from the range's ends), so every partial sum and both
+// multiply-adds equal enc times those of the collision of the d_q, bit for bit, and the result is the
+// value the encode of that collision would store.
 template <class S> struct Collider {
   using R = typename S::real;
   R ux, uy, uz, t15, omt, wr[3], drho, rho;
-  __device__ explicit Collider(const R* fin) {
+  // fin in the form's working representation; dec / enc: the shifted forms' scale (unused otherwise)
+  __device__ Collider(const R* fin, float dec, float enc) {
     const Mom<R> m = moments(fin);
+    R jx = m.ux, jy = m.uy, jz = m.uz;
     if constexpr (S::kShift) {
-      drho = m.rho;
-      rho = R(1) + m.rho;
+      drho = m.rho * R(dec);
+      rho = R(1) + drho;
+      jx = jx * R(dec);
+      jy = jy * R(dec);
+      jz = jz * R(dec);
+      wr[0] = -W<R>(0) * R(enc);
+      wr[1] = -W<R>(1) * R(enc);
+      wr[2] = -W<R>(7) * R(enc);
     } else {
       drho = R(0);
       rho = m.rho;
+      wr[0] = W<R>(0) * rho;
+      wr[1] = W<R>(1) * rho;
+      wr[2] = W<R>(7) * rho;
     }
-    ux = m.ux / rho;
-    uy = m.uy / rho;
-    uz = m.uz / rho;
+    ux = jx / rho;
+    uy = jy / rho;
+    uz = jz / rho;
     t15 = R(1.5) * fmx<R>(uz, uz, fmx<R>(uy, uy, ux * ux));
     omt = R(1) - t15;
-    wr[0] = W<R>(0) * rho;
-    wr[1] = W<R>(1) * rho;
-    wr[2] = W<R>(7) * rho;
   }
-  // the relaxed value of direction q from sc = s_q + cu (plain forms) or g_q + cu (shifted forms)
+  // the relaxed value of direction q from sc = s_q + cu (plain forms) or g_q + cu (shifted forms); wr holds
+  // w rho (plain) or -w enc (shifted) per weight class
   template <int q> __device__ R relax(R x, R om, R sc) const {
     constexpr int cls = q == 0 ? 0 : (q < 7 ? 1 : 2);
-    if constexpr (S::kShift) return fmx<R>(-om, fmx<R>(-W<R>(q), fmx<R>(rho, sc, drho), x), x);
+    if constexpr (S::kShift) return fmx<R>(-om, fmx<R>(wr[cls], fmx<R>(rho, sc, drho), x), x);
     else return fmx<R>(-om, fmx<R>(-wr[cls], sc, x), x);
   }
   template <int q> __device__ R post(R x, R om) const {
@@ -283,6 +300,12 @@ template <class S> struct Collider {
     });
   }
 };
+
+// A population f of direction q (run-time) in the form's working representation, for ghost records.
+template <class S> __device__ inline typename S::real ghost_value(typename S::real f, int q, float enc) {
+  if constexpr (S::kShift) return (f - (q == 0 ? W<float>(0) : (q < 7 ? W<float>(1) : W<float>(7)))) * enc;
+  else return f;
+}
 
 // Runtime-q decode for kernels that gather one direction at a time.
 template <class S> __device__ inline typename S::real load_rt(typename S::raw r, int q, float dec) {

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(BS) void k_dense_step_row(const typename S::raw* __
     if constexpr (kNtLoad) fin[q] = S::template in<q>(__builtin_nontemporal_load(p), dec);
     else fin[q] = S::template in<q>(*p, dec);
   });
-  const Collider<S> col(fin);
+  const Collider<S> col(fin, dec, enc);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const raw v = S::template out<q>(col.template post<q>(fin[q], omega), enc);
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
       if (va[a]) {
         R fin[Q];
         sfor<Q>([&](auto qc) { fin[decltype(qc)::value] = S::template in<decltype(qc)::value>(cur[a][decltype(qc)::value], dec); });
-        Collider<S>(fin).post_all(fin, omega);
+        Collider<S>(fin, dec, enc).post_all(fin, omega);
         sfor<Q>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int d = ring_depth(q);
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
             const int slot = d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1));
             fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * T + c], dec);
           });
-          Collider<S>(fin).post_all(fin, omega);
+          Collider<S>(fin, dec, enc).post_all(fin, omega);
           const uint32_t r = (by * TY + oy) * N + bx * TX + ox;
           sfor<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -457,10 +457,11 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 
 // Column shapes of the fused pair, in order of preference (measured on MI355X at 512^3 FP32: 128 x 4
 // with 832 lanes 58.3 GLUPS, 64 x 8 / 704 lanes 54.3, against 34.2 for single steps; taller columns need
-// two phase-A passes per lane, whose doubled prefetch registers spill at 1024 lanes). FP32 fuses by default.
-// The 16-bit forms move half the bytes, so their fused pass (2.5 collisions per cell where the single step
-// has 1) is bound by arithmetic (FP16 128 x 4: 76 % of the VALU issue slots, against 52 % for FP32): they
-// fuse only when a shape is picked (jp_set_step_pair_shape) or the autotuner keeps one. FP64 would need
+// two phase-A passes per lane, whose doubled prefetch registers spill at 1024 lanes). FP32, FP16 and BF16
+// fuse by default. The 16-bit forms move half the bytes, so their fused pass (2.5 collisions per cell where
+// the single step has 1) is bound by arithmetic rather than bytes; since they collide on the stored scale
+// (no decode or encode multiplies) their pair runs 83.0 / 82.8 GLUPS at 512^3 against 79.5 / 78.9 for
+// their single steps (FP16 / BF16, 128 x 4 columns on the padded z-layers). FP64 would need
 // twice the LDS per cell. Tried and dropped: buffer addressing (one descriptor per z-layer, 32-bit lane
 // offsets) cut the pair's registers from 102 to 69 and allowed two workgroups per CU, but ran no faster
 // (FP32 128 x 4: 63.3 against 64.4 GLUPS with global addresses).
@@ -474,7 +475,6 @@ constexpr int kShapeCount = int(sizeof(kPairShapes) / sizeof(kPairShapes[0]));
 int step_pair_shape_count() { return 2 * kShapeCount; }
 bool step_pair_shape(int prec, int N, int pick, int* out4) {
   if (prec == 3) return false;
-  if (prec != 0 && pick < 0) return false;  // the 16-bit forms fuse on request only (a pick, or jp_autotune)
   // z-segment: the longest of 256, 128, 64, 32 planes that divides N (at 512^3 on padded z-layers 256 planes
   // ran 62.0 against 61.2 GLUPS for 128: two halo planes per segment are 0.8 % of the reads instead of 1.6 %)
   const int tz0 = N % 256 == 0 ? 256 : (N % 128 == 0 ? 128 : (N % 64 == 0 ? 64 : (N % 32 == 0 ? 32 : 0)));

@@ -60,15 +60,16 @@ def test_step_is_bit_exact(prec, n):
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("n", [32, 64, 96])
 def test_fused_step_pairs_are_bit_exact(prec, n):
-    """FP32 lattices whose edge a column shape divides run two fused steps per launch (the other forms run
-    single steps): bit-exact with the oracle over even and odd step counts (an odd count ends with a
+    """FP32, FP16 and BF16 lattices whose edge a column shape divides run two fused steps per launch (FP64
+    runs single steps): bit-exact with the oracle over even and odd step counts (an odd count ends with a
     single step)."""
     o = orc.Uniform(n, re=1600, prec=prec)
     o.init()
     g = jpabi.Uniform(n, re=1600, prec=prec)
     shape = g.step_pair_shape()
     # column x extent chosen per form and edge (0: single steps)
-    expect = {jpabi.PREC_FP32: {32: 32, 64: 64, 96: 32}}
+    fused = {32: 32, 64: 64, 96: 32}
+    expect = {jpabi.PREC_FP32: fused, jpabi.PREC_FP16: fused, jpabi.PREC_BF16: fused}
     assert shape[0] == expect.get(prec, {}).get(n, 0)
     if shape[0]:
         assert n % shape[0] == 0 and n % shape[1] == 0 and n % shape[2] == 0 and shape[3] > 0
@@ -143,13 +144,14 @@ def test_pair_shape_setter_selects_each_shape():
 @pytest.mark.parametrize("prec", [jpabi.PREC_FP16, jpabi.PREC_BF16])
 @pytest.mark.parametrize("pick", [0, 1, 2, 4])
 def test_sixteen_bit_fused_pairs_are_bit_exact(prec, pick):
-    """The 16-bit forms run single steps by default and fuse pairs when a shape is picked: the intermediate
-    step passes through the storage form in LDS, so the fused pass is bit-exact with the oracle."""
+    """The 16-bit forms fuse pairs by default (128 x 4 columns at 128^3) and in each picked shape: the
+    intermediate step passes through the storage form in LDS, so the fused pass is bit-exact with the
+    oracle."""
     n = 128
     o = orc.Uniform(n, re=1600, prec=prec)
     o.init()
     g = jpabi.Uniform(n, re=1600, prec=prec)
-    assert g.step_pair_shape() == (0, 0, 0, 0)
+    assert g.step_pair_shape() == (128, 4, 128, 832)
     g.set_step_pair_shape(pick)
     assert g.step_pair_shape()[0] > 0
     g.set_raw(o.raw())
