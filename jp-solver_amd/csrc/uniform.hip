@@ -464,7 +464,9 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
 // their single steps (FP16 / BF16, 128 x 4 columns on the padded z-layers). FP64 would need
 // twice the LDS per cell. Tried and dropped: buffer addressing (one descriptor per z-layer, 32-bit lane
 // offsets) cut the pair's registers from 102 to 69 and allowed two workgroups per CU, but ran no faster
-// (FP32 128 x 4: 63.3 against 64.4 GLUPS with global addresses).
+// (FP32 128 x 4: 63.3 against 64.4 GLUPS with global addresses); with the direction's offset in a scalar
+// register it also cut 12-14 % of the VALU instructions, and still lost on the issue-bound 16-bit pairs
+// (FP16 81.2 against 82.5, BF16 80.4 against 82.1, same process).
 struct PairShape {
   int tx, ty, bs;
 };
