@@ -62,7 +62,7 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_fill_step_l2(const
   for_tiles<kLoop, false>(dn, [=](int b, int lane) {
     fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, nb1, nb2, slots, G, nqs, dec1, enc2, b, lane, L, ghost);
     __syncthreads();
-    fine_step_cell<S>(fa, fb, ghost, L.nb, om, enc2, dec2, b, lane);
+    fine_step_cell<S, kLoop>(fa, fb, ghost, L.nb, om, enc2, dec2, b, lane);
   });
 }
 
@@ -96,13 +96,13 @@ __global__ __launch_bounds__(512) JP_FULL_OCCUPANCY(S) void k_dense_fill_step_l2
     if (b >= n0) return;
     if (lane < kNbr) L.nb[lane] = nb0[size_t(b) * kNbr + lane];
     __syncthreads();
-    fine_step_cell<S>(c0a, c0b, ghost, L.nb, om0, enc0, dec0, b, lane);
+    fine_step_cell<S, kLoop>(c0a, c0b, ghost, L.nb, om0, enc0, dec0, b, lane);
     return;
   }
   for_tiles_in<kLoop, false>(n2, dn2, int(blockIdx.x) - nd, int(gridDim.x) - nd, [=](int b, int lane) {
     fill_l2_tile<S, 512>(f1, gh2, l2pl1, l2oct, f1pos, nb1, nb2, slots, G, nqs, dec1, enc2, b, lane, L, ghost);
     __syncthreads();
-    fine_step_cell<S>(fa, fb, ghost, L.nb, om2, enc2, dec2, b, lane);
+    fine_step_cell<S, kLoop>(fa, fb, ghost, L.nb, om2, enc2, dec2, b, lane);
   });
 }
 template <class S, bool kLoop>

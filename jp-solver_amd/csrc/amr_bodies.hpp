@@ -219,8 +219,11 @@ __device__ __forceinline__ void pull_source(PullLane P, int k, int& t, uint32_t&
 // c_x = -xe. So a lane needs at most the 8 entries (dz, dy, dx) in {0, ze} x {0, ye} x {0, xe}: it reads
 // them once, and each direction selects among the ones its non-zero components allow (no per-direction
 // LDS lookup and index arithmetic). Element offsets are 32-bit: a fine level holds at most kMaxFineTiles
-// tiles.
-template <class S, bool kKeep = false>
+// tiles. kBitFlags (16-bit forms): the per-direction ghost flags are bits of one vector register instead of
+// lane masks in scalar registers, for the looping fused fill-and-step kernels, whose scalar registers are
+// short (there the 18 masks spilled to vector lanes); a mask costs one select per direction, a bit three
+// instructions.
+template <class S, bool kKeep = false, bool kBitFlags = false>
 __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell, typename S::real (&kept)[Q]) {
@@ -250,10 +253,12 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
     // issued. Decoding inside the branch made each direction wait for its load: 19 round trips in series.
     raw v[Q];
     R gv[Q];
-    uint32_t from_ghost = 0;
+    bool from_ghost[Q];
+    uint32_t ghost_bits = 0;
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       gv[q] = R(0);
+      from_ghost[q] = false;
       if constexpr (q == 0) {
         v[0] = fa[tidx(b, 0, cell)];
       } else {
@@ -262,9 +267,10 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
         pull_source<q>(P, k, t, src);
         // a ghost lane reads the same in-tile index of its own tile (in bounds, discarded)
         v[q] = fa[__umul24(uint32_t(t >= 0 ? t : b), uint32_t(Q * MB3)) + src];
+        if constexpr (!kBitFlags) from_ghost[q] = t < 0;
         if (t < 0) {
           gv[q] = g[ghost_slot(q, i, j, k)];
-          from_ghost |= 1u << q;
+          if constexpr (kBitFlags) ghost_bits |= 1u << q;
         }
       }
     });
@@ -274,19 +280,20 @@ __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict
       // into the branch above, where it would wait for the load
       R gq = gv[q];
       asm volatile("" : "+v"(gq));
-      fin[q] = (from_ghost >> q) & 1u ? gq : S::template in<q>(v[q], dec);
+      const bool ghost = kBitFlags ? ((ghost_bits >> q) & 1u) != 0 : from_ghost[q];
+      fin[q] = ghost ? gq : S::template in<q>(v[q], dec);
     });
   }
   collide_store<S, kKeep>(fin, om, fb, b, cell, enc, kept, dec);
 }
 
 // The same without keeping the stored values.
-template <class S>
+template <class S, bool kBitFlags = false>
 __device__ __forceinline__ void fine_step_cell(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                       const typename S::real* __restrict__ g, const int* snb, typename S::real om,
                                       float enc, float dec, int b, int cell) {
   typename S::real none[Q];
-  fine_step_cell<S, false>(fa, fb, g, snb, om, enc, dec, b, cell, none);
+  fine_step_cell<S, false, kBitFlags>(fa, fb, g, snb, om, enc, dec, b, cell, none);
 }
 
 // Last fine substep with the restriction fused in, over one slab of sibling tiles. The 8 fine tiles of a
@@ -305,7 +312,7 @@ template <class R> struct RestrictLds {
   R part[4][Q][16];     // partial sums per sibling, direction, parent of the sibling's 4 x 4 layer
   R avg[Q][64];         // complete child sums per direction and parent (py * 8 + px)
 };
-template <class S>
+template <class S, bool kBitFlags = false>
 __device__ __forceinline__ void fine_step_restrict(const typename S::raw* __restrict__ fa, typename S::raw* __restrict__ fb,
                                           const typename S::real* __restrict__ gh, const int* __restrict__ nb,
                                           const int* __restrict__ ptile, int u, int lane, typename S::real om, float enc,
@@ -318,7 +325,7 @@ __device__ __forceinline__ void fine_step_restrict(const typename S::raw* __rest
   __syncthreads();
   const int b = t0 + sib, cell = (2 * (u & 3) + (w & 1)) * 64 + l;
   R kept[Q];
-  fine_step_cell<S, true>(fa, fb, gh + size_t(b) * kGhostSlots, L.nb + sib * kNbr, om, enc, dec, b, cell, kept);
+  fine_step_cell<S, true, kBitFlags>(fa, fb, gh + size_t(b) * kGhostSlots, L.nb + sib * kNbr, om, enc, dec, b, cell, kept);
   const int i = l & 7, j = l >> 3;
   const bool corner = !(i & 1) && !(j & 1);
   const int pp = (j >> 1) * 4 + (i >> 1);
