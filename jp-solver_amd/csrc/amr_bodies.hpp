@@ -148,7 +148,8 @@ __device__ __forceinline__ void restrict_finish(typename S::real (&avg)[Q], type
   using R = typename S::real;
   R nq[Q];
   const Mom<R> m = moments(avg);
-  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  R ux = m.ux, uy = m.uy, uz = m.uz;
+  quot3(m.rho, ux, uy, uz);
   const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -357,7 +358,8 @@ __device__ __forceinline__ void fine_step_restrict(const typename S::raw* __rest
 template <class R> __device__ __forceinline__ void prep_values(R* v, float nqs) {
   R nq[Q];
   const Mom<R> m = moments(v);
-  const R ux = m.ux / m.rho, uy = m.uy / m.rho, uz = m.uz / m.rho;
+  R ux = m.ux, uy = m.uy, uz = m.uz;
+  quot3(m.rho, ux, uy, uz);
   const EqPre<R> e = eq_pre<R>(m.rho, ux, uy, uz);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;

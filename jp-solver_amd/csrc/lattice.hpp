@@ -223,6 +223,35 @@ template <class R> __device__ __forceinline__ R fmx(R a, R b, R c) {
   else return __builtin_fma(a, b, c);
 }
 
+// The three velocity quotients u = j / rho of a cell, each the correctly rounded quotient (IEEE division),
+// from one reciprocal instead of three division sequences (10 instructions each, 2 of them serialised on
+// VCC). Binary32: Y0 = v_rcp_f32(b) (relative error <= 2^-23; the argument below holds for any start within
+// 2^-13), two Newton steps in binary64, Y <- Y + Y (1 - b Y), both products exact inside the fma, leave
+// |Y b - 1| <= 2^-53 (1 + 2^-38); Q = RN64(a Y) is then within 2^-52 (1 + 2^-38) |a / b| of a / b. A float
+// quotient a / b never lies within 2^-49 |a / b| of a binary32 rounding boundary (a midpoint m 2^e with m
+// odd, 2^24 <= m < 2^25): with a = A 2^s, b = B 2^t, |A|, |B| < 2^24, the distance is
+// |A 2^s - B m 2^(t+e)| / |b|, whose numerator is a non-zero multiple of 2^min(s, t + e) (equality would
+// need odd(A) = odd(B) m >= 2^24), i.e. at least 2^e / |B| > |m 2^e| 2^-49 or |a / b| 2^-24. So RN32(Q) =
+// RN32(a / b) bit for bit, zero signs included (a = +-0 gives +-0), for every finite a and normal b > 0
+// whose quotient is a normal float (rho is near 1 and |j| < 1 here). Binary64 keeps IEEE division.
+template <class R> __device__ __forceinline__ void quot3(R b, R& x, R& y, R& z) {
+  if constexpr (sizeof(R) == 4) {
+    const double B = double(b);
+    double Y = double(__builtin_amdgcn_rcpf(b));
+    double E = __builtin_fma(-B, Y, 1.0);
+    Y = __builtin_fma(Y, E, Y);
+    E = __builtin_fma(-B, Y, 1.0);
+    Y = __builtin_fma(Y, E, Y);
+    x = float(double(x) * Y);
+    y = float(double(y) * Y);
+    z = float(double(z) * Y);
+  } else {
+    x = x / b;
+    y = y / b;
+    z = z / b;
+  }
+}
+
 // BGK collision of one cell in the form's working representation (src/tgv3d_uniform.cu:55-60,
 // src/tgv3d_amr.cu:268-273): post<q>(x) = x - omega (x - feq_q), feq_q = w_q rho (1 + cu + cu^2/2 - 1.5 u^2),
 // cu = 3 c_q.u, with the multiply-adds fused at fixed points (the reference's compiler contracts them too,
@@ -263,9 +292,10 @@ template <class S> struct Collider {
       wr[1] = W<R>(1) * rho;
       wr[2] = W<R>(7) * rho;
     }
-    ux = jx / rho;
-    uy = jy / rho;
-    uz = jz / rho;
+    ux = jx;
+    uy = jy;
+    uz = jz;
+    quot3(rho, ux, uy, uz);
     t15 = R(1.5) * fmx<R>(uz, uz, fmx<R>(uy, uy, ux * ux));
     omt = R(1) - t15;
   }
