@@ -324,18 +324,25 @@ __global__ __launch_bounds__(BS) void k_dense_step_vec(const typename S::raw* __
 // is bit-identical to two single steps.
 //
 // LDS holds each direction's intermediate planes as a ring: the second step at plane o pulls direction q
-// from plane o - c_z, and the newest intermediate plane is o + 1, so a direction with c_z = -1 needs 1
-// plane, c_z = 0 needs 2 and c_z = +1 needs 3: 38 planes in all. A ring plane of direction q holds only
-// the TX x TY values the second step pulls, indexed by the output cell that pulls them: the extended
-// cell (ex, ey) feeds direction q of output cell (ex - 1 + c_x, ey - 1 + c_y) and nothing else, so phase
-// A stores each value at most once and phase B reads every direction at its own cell index.
-__host__ __device__ constexpr int ring_depth(int q) { return 2 + CZ(q); }
-__host__ __device__ constexpr int ring_base(int q) {
+// from plane o - c_z, and the newest intermediate plane is o + 1, so phase B of a plane reads the newest
+// plane of a direction with c_z = -1, the second newest of one with c_z = 0 and the third newest of one with
+// c_z = +1. Rings of 1, 2 and 3 planes (38 in all) need a second barrier after phase B, before the next
+// phase A overwrites a slot phase B reads. The 16-bit forms keep one plane more per ring (2, 3 and 4, 57 in
+// all, 58.4 KB): phase A of the next plane then writes slots no phase B of this plane reads, one barrier
+// per plane (between A and B) orders the rings, and the waves with no phase-B cell (832 lanes, 512 output
+// cells) go on to the next plane's phase A while the others finish phase B (512^3: FP16 85.0-87.4 ->
+// 87.8-88.2 GLUPS, BF16 85.4-86.2 -> 90.0). The FP32 pair, which waits on memory more than on issue, ran
+// 1-2 % slower so (63.4-63.5 against 64.4-64.7 GLUPS) and keeps two barriers.
+// A ring plane of direction q holds only the TX x TY values the second step pulls, indexed by the output
+// cell that pulls them: the extended cell (ex, ey) feeds direction q of output cell (ex - 1 + c_x,
+// ey - 1 + c_y) and nothing else, so phase A stores each value at most once and phase B reads every
+// direction at its own cell index.
+__host__ __device__ constexpr int ring_depth(int q, int extra) { return 2 + extra + CZ(q); }
+__host__ __device__ constexpr int ring_base(int q, int extra) {
   int s = 0;
-  for (int i = 0; i < q; ++i) s += ring_depth(i);
+  for (int i = 0; i < q; ++i) s += ring_depth(i, extra);
   return s;
 }
-constexpr int kRingPlanes = ring_base(Q);
 
 // Workgroup barrier that orders LDS only: the population loads of the next plane stay in flight across it.
 __device__ __forceinline__ void lds_barrier() {
@@ -352,7 +359,8 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
   using R = typename S::real;
   constexpr int EX = TX + 2, EY = TY + 2, E = EX * EY, T = TX * TY;
   constexpr int PA = (E + BS - 1) / BS, PB = (T + BS - 1) / BS;
-  __shared__ raw ring[kRingPlanes * T];
+  constexpr int X = sizeof(raw) == 2 ? 1 : 0;  // extra ring plane per direction: one barrier per plane
+  __shared__ raw ring[ring_base(Q, X) * T];
   const uint32_t N = g.N;
   const uint32_t ntx = N / TX, nty = N / TY, nt = ntx * nty * (N / TZ);
   // every XCD streams a contiguous run of columns, so halo rows shared by neighbouring columns are read
@@ -398,8 +406,8 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
       });
   };
   auto body = [&](const raw(&cur)[PA][Q], int it) {
-    const int m2 = it & 1, m3 = it % 3;
-    // phase A: intermediate plane it of the extended tile
+    const int m2 = it & 1, m3 = it % 3, m4 = it & 3;
+    // phase A: intermediate plane it of the extended tile, into slot it mod depth
 #pragma unroll
     for (int a = 0; a < PA; ++a) {
       if (va[a]) {
@@ -408,11 +416,11 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
         Collider<S>(fin, dec, enc).post_all(fin, omega);
         sfor<Q>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
-          constexpr int d = ring_depth(q);
-          const int slot = d == 1 ? 0 : (d == 2 ? m2 : m3);
+          constexpr int d = ring_depth(q, X);
+          const int slot = d == 1 ? 0 : (d == 2 ? m2 : (d == 3 ? m3 : m4));
           const int ox = exa[a] - 1 + CX(q), oy = eya[a] - 1 + CY(q);  // the output cell pulling q from here
           if (unsigned(ox) < unsigned(TX) && unsigned(oy) < unsigned(TY))
-            ring[(ring_base(q) + slot) * T + oy * TX + ox] = S::template out<q>(fin[q], enc);
+            ring[(ring_base(q, X) + slot) * T + oy * TX + ox] = S::template out<q>(fin[q], enc);
         });
       }
     }
@@ -428,10 +436,11 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
           R fin[Q];
           sfor<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            constexpr int d = ring_depth(q);
-            // plane o - c_z: c_z = -1 the newest (slot of it), 0 that of it - 1, +1 that of it - 2
-            const int slot = d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1));
-            fin[q] = S::template in<q>(ring[(ring_base(q) + slot) * T + c], dec);
+            constexpr int d = ring_depth(q, X);
+            // plane o - c_z: c_z = -1 the newest (that of it), 0 that of it - 1, +1 that of it - 2
+            const int slot = X == 0 ? (d == 1 ? 0 : (d == 2 ? (m2 ^ 1) : (m3 == 2 ? 0 : m3 + 1)))
+                                    : (d == 2 ? m2 : (d == 3 ? (m3 == 0 ? 2 : m3 - 1) : (m4 ^ 2)));
+            fin[q] = S::template in<q>(ring[(ring_base(q, X) + slot) * T + c], dec);
           });
           Collider<S>(fin, dec, enc).post_all(fin, omega);
           const uint32_t r = (by * TY + oy) * N + bx * TX + ox;
@@ -442,7 +451,7 @@ __global__ __launch_bounds__(BS) void k_dense_step2(const typename S::raw* __res
         }
       }
     }
-    lds_barrier();
+    if constexpr (X == 0) lds_barrier();
   };
   // intermediate planes 0 .. TZ + 1 in pairs (TZ is even); the last prefetch repeats plane TZ + 1
   raw ra[PA][Q], rb[PA][Q];
