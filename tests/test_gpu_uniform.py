@@ -161,6 +161,24 @@ def test_sixteen_bit_fused_pairs_are_bit_exact(prec, pick):
     g.close()
 
 
+@pytest.mark.parametrize("prec", [jpabi.PREC_FP16, jpabi.PREC_BF16])
+@pytest.mark.parametrize("n", [32, 96])
+def test_sixteen_bit_pairs_on_short_segments(prec, n):
+    """The 16-bit pairs keep one more ring plane per direction and a single barrier per plane; at edges 32
+    and 96 the only column is 32 x 16 over 32-plane segments, so every ring slot turns over within a segment
+    and the pass from one segment to the next (z periodic at 32) is exercised. Bit-exact with the oracle
+    over three pairs and a single step."""
+    o = orc.Uniform(n, re=1600, prec=prec)
+    o.init()
+    g = jpabi.Uniform(n, re=1600, prec=prec)
+    assert g.step_pair_shape() == (32, 16, 32, 640)
+    g.set_raw(o.raw())
+    o.step(7)
+    g.step(7)
+    assert np.array_equal(g.raw(), o.raw())
+    g.close()
+
+
 @pytest.mark.parametrize("n,prec", [(256, jpabi.PREC_FP32), (96, jpabi.PREC_FP32), (64, jpabi.PREC_FP16)])
 def test_autotune_keeps_state_and_picks_a_valid_form(n, prec):
     """The uniform autotuner times single steps in its launch variants and the fused column shapes that
