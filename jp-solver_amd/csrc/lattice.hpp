@@ -225,9 +225,10 @@ template <class R> __device__ __forceinline__ R fmx(R a, R b, R c) {
 
 // The three velocity quotients u = j / rho of a cell, each the correctly rounded quotient (IEEE division),
 // from one reciprocal instead of three division sequences (10 instructions each, 2 of them serialised on
-// VCC). Binary32: Y0 = v_rcp_f32(b) (relative error <= 2^-23; the argument below holds for any start within
-// 2^-13), two Newton steps in binary64, Y <- Y + Y (1 - b Y), both products exact inside the fma, leave
-// |Y b - 1| <= 2^-53 (1 + 2^-38); Q = RN64(a Y) is then within 2^-52 (1 + 2^-38) |a / b| of a / b. A float
+// VCC). Binary32: Y0 = v_rcp_f32(b) (relative error at most 2^-23). Each Newton step in binary64,
+// Y <- fma(Y, fma(-b, Y, 1), Y), squares the relative error and adds one rounding (2^-53), so two steps
+// from any start within 2^-13 leave |Y b - 1| <= 1.5 2^-52 (from v_rcp_f32's start, 2^-53 (1 + 2^-38)), and
+// Q = RN64(a Y) lies within 2^-51 |a / b| of a / b. A float
 // quotient a / b never lies within 2^-49 |a / b| of a binary32 rounding boundary (a midpoint m 2^e with m
 // odd, 2^24 <= m < 2^25): with a = A 2^s, b = B 2^t, |A|, |B| < 2^24, the distance is
 // |A 2^s - B m 2^(t+e)| / |b|, whose numerator is a non-zero multiple of 2^min(s, t + e) (equality would
